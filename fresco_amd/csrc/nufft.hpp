@@ -1,0 +1,135 @@
+// Non-uniform FFT building blocks on gfx950:
+//   * SortedPoints : coordinate map -> fine-grid cell + offset, bin (tile) key,
+//                    stable radix sort by bin, bin offsets, sorted point records.
+//   * Spreader     : type-1 spreading, owner-computes tiles accumulated in LDS
+//                    (each fine-grid cell is written exactly once, no atomics,
+//                    fixed summation order -> bitwise deterministic).
+//   * interpolate  : type-2 interpolation, one wavefront per point with a
+//                    wavefront reduction; either the value or the gradient
+//                    through the kernel derivative.
+//   * FftPlan      : rocFFT plan + work buffer + execution info.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "es_kernel.hpp"
+
+struct rocfft_plan_t;
+struct rocfft_execution_info_t;
+
+namespace fresco_hip {
+
+template <typename T>
+struct Cplx {
+  T re, im;
+};
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+// Coordinate map applied to each component of a point before it is placed on
+// the fine grid:  wrap=1: x = (p - round(p/L) + shift) * scale   (the
+// reference's map, nuwu.hpp:160,219);  wrap=0: x = p * scale.
+// The fine-grid coordinate is then g = frac(x / 2pi) * nf.
+struct CoordMap {
+  int wrap = 1;
+  double L[3] = {1, 1, 1};
+  double shift[3] = {0, 0, 0};
+  double scale[3] = {1, 1, 1};
+};
+
+struct GridGeom {
+  int d = 3;
+  int nf[3] = {1, 1, 1};     // fine grid cells per dimension
+  int tile[3] = {1, 1, 1};   // spreader tile (= bin) size per dimension
+  int ntile[3] = {1, 1, 1};  // tiles per dimension
+  int nbins = 1;
+  int key_bits = 1;
+  int cell_bits = 10;  // bits per dimension of the packed first-cell word
+  int64_t ncells() const { return int64_t(nf[0]) * nf[1] * nf[2]; }
+};
+
+// Choose tiles for a fine grid (power-of-two tiles, enough of them to fill
+// the GPU, LDS budget respected).
+GridGeom make_geom(int d, const int* nf, int w, int elem_bytes);
+
+// Sorted non-uniform point set shared by the spreader and the interpolator.
+template <typename T>
+class SortedPoints {
+ public:
+  using R4 = typename Vec4<T>::type;
+  void init(const GridGeom& g, const EsParams& es, int64_t n);
+  // x: device float64 [n*d]; c: device T[n] weights (may be null -> 1).
+  void set(const double* x, const CoordMap& m, const T* c, hipStream_t s);
+
+  int64_t n = 0;
+  GridGeom geom;
+  EsParams es;
+  DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
+  DeviceBuffer<int> bin_start;                          // [nbins + 1]
+  DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
+  DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
+  DeviceBuffer<unsigned char> sort_tmp;
+  size_t sort_tmp_bytes = 0;
+};
+
+template <typename T>
+class Spreader {
+ public:
+  void init(const GridGeom& g, const EsParams& es);
+  // grid: device T[ncells], fully overwritten.
+  void spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
+
+  GridGeom geom;
+  EsParams es;
+  int maxc[3] = {1, 1, 1};
+  DeviceBuffer<int> cand;  // [3][maxtile][MAXC] candidate bins per tile, -1 padded
+  int cand_stride = 0;     // entries per dimension block
+  int lds_bytes = 0;
+  int nwaves = 4;
+};
+
+// Interpolation of a real fine grid at the sorted points.
+//  deriv=false: out[idx*ostride + ocomp] = c * sum_l b_l psi(l - g)
+//  deriv=true : out[idx*ostride + a] = c * dscale[a] * sum_l b_l dpsi_a(l - g), a < d
+template <typename T>
+void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const double* dscale,
+                 double* out, int ostride, int ocomp, hipStream_t s);
+
+// rocFFT plan for one transform on a d-dimensional grid (lengths fastest first).
+class FftPlan {
+ public:
+  enum Kind { R2C, C2R, C2C_FWD, C2C_INV };
+  FftPlan() = default;
+  ~FftPlan();
+  FftPlan(const FftPlan&) = delete;
+  FftPlan& operator=(const FftPlan&) = delete;
+  void create(Kind k, bool dbl, int d, const int* lengths_fastest_first);
+  void exec(void* in, void* out, hipStream_t s);
+
+ private:
+  rocfft_plan_t* plan_ = nullptr;
+  rocfft_execution_info_t* info_ = nullptr;
+  DeviceBuffer<unsigned char> work_;
+};
+
+// out[i] = 0 over n elements of T (graph-capture safe fill).
+template <typename T>
+void fill_zero(T* p, size_t n, hipStream_t s);
+
+// Deterministic FP64 sum of partials[n] -> *out (device), Kahan-compensated.
+void reduce_partials(const double* partials, int n, double* out, hipStream_t s);
+
+}  // namespace fresco_hip

@@ -1,0 +1,439 @@
+// NUwU: structure-factor loss of non-uniform points on a uniform k grid.
+//
+// Reference: source/fresco/nuwu.hpp (constructor 35-52, mode grid 54-79,
+// weights 125-149, energy 152-208, energy+gradient 210-317).
+//
+// Per evaluation (E+grad):
+//   bin/sort points -> spread (real fine grid, weights c real) -> R2C FFT ->
+//   fused deconvolve + loss + pullback over the active (+-k) mode pairs ->
+//   C2R FFT -> interpolation with the kernel derivative (all d gradient
+//   components from one grid).
+// The FP64 "plain" variant keeps the reference's structure of d separate
+// type-2 transforms (d C2R FFTs + d value interpolations).
+//
+// Because c is real, rho_{-k} = conj(rho_k): the loss visits each pair
+// {k, -k} once, from the half spectrum of the real-to-complex FFT.  Modes whose
+// weight V is 0 contribute exactly 0 to the energy and the gradient
+// (nuwu.hpp:271-272) and are skipped.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "nufft.hpp"
+#include "potential.hpp"
+
+namespace fresco_hip {
+
+namespace {
+
+struct PairDev {
+  const int* h;      // half-spectrum index of k (kx >= 0 representative)
+  const int* hneg;   // half-spectrum index of -k when kx == 0 and k != 0, else -1
+  const int* ik;     // flat mode index of k (reference order)
+  const int* imk;    // flat mode index of -k
+  const int* kv;     // k components [npairs*3]
+  const double* Vk;
+  const double* Vm;
+  const double* Sk;
+  const double* Sm;
+  const double* ipsi;  // 1 / Psi(k)
+};
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256)
+    k_nuwu_loss(const Cplx<T>* __restrict__ S, PairDev pd, int npairs, double norm, double gscale,
+                Cplx<T>* __restrict__ X0, Cplx<T>* __restrict__ X1, Cplx<T>* __restrict__ X2, int d,
+                double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double e = 0.0;
+  if (i < npairs) {
+    const Cplx<T> sv = S[pd.h[i]];
+    const double ip = pd.ipsi[i];
+    const double re = (double)sv.re * ip;   // rho_k = conj(S_k) / Psi(k)
+    const double im = -(double)sv.im * ip;
+    const double s = (re * re + im * im) / norm;  // nuwu.hpp:262-264
+    const double Sk = pd.Sk[i], Vk = pd.Vk[i];
+    double D = s - Sk;
+    if (Sk != 0) D /= Sk;  // nuwu.hpp:265-268
+    e = Vk * D * D;        // nuwu.hpp:270-271
+    double gk = Vk * D;
+    if (Sk != 0) gk /= Sk;  // nuwu.hpp:272-276
+    const bool self = pd.ik[i] == pd.imk[i];
+    double gm = gk;
+    if (!self) {
+      const double Sm = pd.Sm[i], Vm = pd.Vm[i];
+      double Dm = s - Sm;
+      if (Sm != 0) Dm /= Sm;
+      e += Vm * Dm * Dm;
+      gm = Vm * Dm;
+      if (Sm != 0) gm /= Sm;
+    }
+    if (MODE > 0) {
+      // H_k = (factor_k + conj(factor_-k)) / 2 with factor = gscale * g * rho
+      double hr, hi;
+      if (self) {
+        hr = gscale * gk * re;
+        hi = 0.0;
+      } else {
+        const double gg = 0.5 * gscale * (gk + gm);
+        hr = gg * re;
+        hi = gg * im;
+      }
+      // C2R input X_k = conj(H_k) / Psi(k): the backward FFT then yields
+      // sum_k H_k / Psi(k) e^{-i k.l h} (type-2 sign -1, nuwu.hpp:51).
+      const double xr = hr * ip, xi = -hi * ip;
+      const int h = pd.h[i], hn = pd.hneg[i];
+      if (MODE == 1) {
+        X0[h] = Cplx<T>{(T)xr, (T)xi};
+        if (hn >= 0) X0[hn] = Cplx<T>{(T)xr, (T)(-xi)};
+      } else {
+        // f_a = -i k_a factor  =>  X_a = i k_a X
+        Cplx<T>* Xs[3] = {X0, X1, X2};
+        for (int a = 0; a < d; ++a) {
+          const double ka = (double)pd.kv[3 * i + a];
+          const double yr = -ka * xi, yi = ka * xr;
+          Xs[a][h] = Cplx<T>{(T)yr, (T)yi};
+          if (hn >= 0) Xs[a][hn] = Cplx<T>{(T)yr, (T)(-yi)};
+        }
+      }
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int hh = 128; hh >= 1; hh >>= 1) {
+    if ((int)threadIdx.x < hh) red[threadIdx.x] += red[threadIdx.x + hh];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+struct RhoDev {
+  int d, kp, n;
+  int nf[3];
+  int hx;  // nf[0]/2 + 1
+  const double* psi0;
+  const double* psi1;
+  const double* psi2;
+};
+
+template <typename T>
+__global__ void k_nuwu_rho(const Cplx<T>* __restrict__ S, RhoDev r, int64_t m, double* __restrict__ rho) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int k[3] = {0, 0, 0};
+  k[0] = (int)(i % r.n) - r.kp;
+  if (r.d >= 2) k[1] = (int)((i / r.n) % r.n) - r.kp;
+  if (r.d >= 3) k[2] = (int)(i / ((int64_t)r.n * r.n)) - r.kp;
+  bool rep = k[0] > 0 || (k[0] == 0 && (k[1] > 0 || (k[1] == 0 && k[2] >= 0)));
+  int q[3];
+  for (int a = 0; a < 3; ++a) q[a] = rep ? k[a] : -k[a];
+  int hy = q[1] < 0 ? q[1] + r.nf[1] : q[1];
+  int hz = q[2] < 0 ? q[2] + r.nf[2] : q[2];
+  int64_t h = q[0] + (int64_t)r.hx * (hy + (int64_t)r.nf[1] * hz);
+  double ps = r.psi0[abs(k[0])];
+  if (r.d >= 2) ps *= r.psi1[abs(k[1])];
+  if (r.d >= 3) ps *= r.psi2[abs(k[2])];
+  Cplx<T> sv = S[h];
+  double re = (double)sv.re / ps, im = (double)sv.im / ps;
+  // rep: rho = conj(S)/Psi;  mirrored: rho_k = conj(rho_-k) = S_-k / Psi
+  rho[2 * i] = re;
+  rho[2 * i + 1] = rep ? -im : im;
+}
+
+int kvec_size(int64_t m, int d) {
+  // calculate_Kvec (nuwu.hpp:54-79)
+  if (d == 1) return (int)((m - 1) / 2);
+  if (d == 2) return (int)((std::sqrt((double)m) - 1.0) / 2.0);
+  if (d == 3) return (int)((std::cbrt((double)m) - 1.0) / 2.0);
+  throw RuntimeFailure("Invalid dimension: " + std::to_string(d));
+}
+
+template <typename T>
+class NUwU final : public Potential {
+ public:
+  NUwU(const double* radii, int64_t n, int d, double K, const double* Sk, const double* V, int64_t m,
+       const double* L, double eps, int precision, double sigma)
+      : N_(n), d_(d), plain_(precision == 0) {
+    (void)K;  // stored but unused by the arithmetic (nuwu.hpp:23,39)
+    if (d < 1 || d > 3) throw InvalidArgument("Invalid dimension: " + std::to_string(d));
+    if (n <= 0) throw InvalidArgument("radii must be non-empty");
+    if (m <= 0) throw InvalidArgument("Sk / V must be non-empty");
+    kp_ = kvec_size(m, d);
+    nm_ = 2 * kp_ + 1;
+    int64_t mm = 1;
+    for (int a = 0; a < d; ++a) mm *= nm_;
+    if (mm != m)
+      throw InvalidArgument("V.size() = " + std::to_string(m) + " is not (2K+1)^d; nearest grid is " +
+                            std::to_string(nm_) + "^" + std::to_string(d));
+    M_ = m;
+    for (int a = 0; a < d; ++a) {
+      if (!(L[a] > 0)) throw InvalidArgument("box lengths must be positive");
+      L_[a] = L[a];
+    }
+    if (sigma <= 0) sigma = 2.0;
+    es_ = es_params(eps, sigma);
+    int nf[3] = {1, 1, 1};
+    for (int a = 0; a < d; ++a) nf[a] = next_smooth_even((int)std::ceil(sigma * nm_), es_.w);
+    geom_ = make_geom(d, nf, es_.w, (int)sizeof(T));
+
+    // weights c (nuwu.hpp:125-149)
+    std::vector<double> c(n, 1.0);
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      for (int a = 0; a < d; ++a) {
+        if (d == 2) c[i] = c[i] * M_PI * radii[i] * radii[i];
+        else if (d == 3) c[i] = c[i] * M_PI * radii[i] * radii[i] * radii[i] * 4.0 / 3.0;
+      }
+      sum += c[i];
+    }
+    std::vector<T> ct(n);
+    for (int64_t i = 0; i < n; ++i) ct[i] = (T)(c[i] * ((double)n / sum));
+    c_.allocate(n);
+    c_.upload(ct.data(), n);
+
+    pts_.init(geom_, es_, n);
+    spreader_.init(geom_, es_);
+    grid_.allocate(geom_.ncells());
+    hx_ = geom_.nf[0] / 2 + 1;
+    nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
+    S_.allocate(nspec_);
+    nX_ = plain_ ? d : 1;
+    for (int a = 0; a < nX_; ++a) X_[a].allocate(nspec_);
+    const bool dbl = sizeof(T) == 8;
+    r2c_.create(FftPlan::R2C, dbl, d, geom_.nf);
+    c2r_.create(FftPlan::C2R, dbl, d, geom_.nf);
+
+    // deconvolution factors per dimension
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> ps = a < d ? es_deconv(es_, geom_.nf[a], kp_) : std::vector<double>(kp_ + 1, 1.0);
+      psi_host_[a] = ps;
+      psi_[a].allocate(ps.size());
+      psi_[a].upload(ps.data(), ps.size());
+    }
+    build_pairs(Sk, V);
+
+    // coordinate maps: energy path (nuwu.hpp:160) has +L, gradient path (219) not
+    for (int a = 0; a < 3; ++a) {
+      double La = a < d ? L_[a] : 1.0;
+      map_e_.L[a] = map_g_.L[a] = La;
+      map_e_.shift[a] = La;
+      map_g_.shift[a] = 0.0;
+      map_e_.scale[a] = map_g_.scale[a] = 2.0 * M_PI / La;
+      dscale_[a] = a < d ? -(2.0 / es_.w) * (geom_.nf[a] / (2.0 * M_PI)) : 0.0;
+    }
+  }
+
+  int64_t ndof() const override { return N_ * d_; }
+  int64_t nmodes() const override { return M_; }
+
+  double energy(const double* x, hipStream_t s) override {
+    {
+      ProfScope p(ST_SORT, s);
+      pts_.set(x, map_e_, c_.get(), s);
+    }
+    {
+      ProfScope p(ST_SPREAD, s);
+      spreader_.spread(pts_, grid_.get(), s);
+    }
+    {
+      ProfScope p(ST_FFT_FWD, s);
+      r2c_.exec(grid_.get(), S_.get(), s);
+    }
+    {
+      ProfScope p(ST_LOSS, s);
+      launch_loss<0>(s);
+      reduce_partials(partial_.get(), nblk_, e_dev_.get(), s);
+    }
+    return host_.fetch(e_dev_.get(), s);
+  }
+
+  double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    {
+      ProfScope p(ST_SORT, s);
+      pts_.set(x, map_g_, c_.get(), s);
+    }
+    {
+      ProfScope p(ST_SPREAD, s);
+      spreader_.spread(pts_, grid_.get(), s);
+    }
+    {
+      ProfScope p(ST_FFT_FWD, s);
+      r2c_.exec(grid_.get(), S_.get(), s);
+    }
+    {
+      ProfScope p(ST_ZERO, s);
+      for (int a = 0; a < nX_; ++a) fill_zero(X_[a].get(), nspec_, s);
+    }
+    {
+      ProfScope p(ST_LOSS, s);
+      if (plain_)
+        launch_loss<2>(s);
+      else
+        launch_loss<1>(s);
+      reduce_partials(partial_.get(), nblk_, e_dev_.get(), s);
+    }
+    if (plain_) {
+      for (int a = 0; a < d_; ++a) {
+        {
+          ProfScope p(ST_FFT_INV, s);
+          c2r_.exec(X_[a].get(), grid_.get(), s);
+        }
+        ProfScope p(ST_INTERP, s);
+        interpolate(pts_, grid_.get(), false, nullptr, grad, d_, a, s);
+      }
+    } else {
+      {
+        ProfScope p(ST_FFT_INV, s);
+        c2r_.exec(X_[0].get(), grid_.get(), s);
+      }
+      ProfScope p(ST_INTERP, s);
+      interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
+    }
+    return host_.fetch(e_dev_.get(), s);
+  }
+
+  void get_rho(double* rho, hipStream_t s) override {
+    RhoDev r;
+    r.d = d_;
+    r.kp = kp_;
+    r.n = nm_;
+    for (int a = 0; a < 3; ++a) r.nf[a] = geom_.nf[a];
+    r.hx = hx_;
+    r.psi0 = psi_[0].get();
+    r.psi1 = psi_[1].get();
+    r.psi2 = psi_[2].get();
+    hipLaunchKernelGGL(k_nuwu_rho<T>, dim3(blocks_for(M_, 256)), dim3(256), 0, s, S_.get(), r, M_, rho);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+
+  void plan_info(int64_t* nf, int* w, double* beta, int* tile) const override {
+    for (int a = 0; a < 3; ++a) {
+      nf[a] = geom_.nf[a];
+      tile[a] = geom_.tile[a];
+    }
+    *w = es_.w;
+    *beta = es_.beta;
+  }
+
+ private:
+  template <int MODE>
+  void launch_loss(hipStream_t s) {
+    Cplx<T>* X0 = X_[0].get();
+    Cplx<T>* X1 = nX_ > 1 ? X_[1].get() : nullptr;
+    Cplx<T>* X2 = nX_ > 2 ? X_[2].get() : nullptr;
+    hipLaunchKernelGGL((k_nuwu_loss<T, MODE>), dim3(nblk_), dim3(256), 0, s, S_.get(), pd_, npairs_,
+                       (double)N_, 4.0 / (double)N_, X0, X1, X2, d_, partial_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+
+  void build_pairs(const double* Sk, const double* V) {
+    std::vector<int> h, hn, ik, imk, kv;
+    std::vector<double> vk, vm, sk, sm, ip;
+    const int n = nm_;
+    for (int64_t i = 0; i < M_; ++i) {
+      int k[3] = {0, 0, 0};
+      k[0] = (int)(i % n) - kp_;
+      if (d_ >= 2) k[1] = (int)((i / n) % n) - kp_;
+      if (d_ >= 3) k[2] = (int)(i / ((int64_t)n * n)) - kp_;
+      bool rep = k[0] > 0 || (k[0] == 0 && (k[1] > 0 || (k[1] == 0 && k[2] >= 0)));
+      if (!rep) continue;
+      int64_t j = (int64_t)(-k[0] + kp_);
+      if (d_ >= 2) j += (int64_t)n * (-k[1] + kp_);
+      if (d_ >= 3) j += (int64_t)n * n * (-k[2] + kp_);
+      if (V[i] == 0.0 && V[j] == 0.0) continue;
+      int hy = k[1] < 0 ? k[1] + geom_.nf[1] : k[1];
+      int hz = k[2] < 0 ? k[2] + geom_.nf[2] : k[2];
+      h.push_back((int)(k[0] + (int64_t)hx_ * (hy + (int64_t)geom_.nf[1] * hz)));
+      if (k[0] == 0 && j != i) {
+        int ny = -k[1] < 0 ? -k[1] + geom_.nf[1] : -k[1];
+        int nz = -k[2] < 0 ? -k[2] + geom_.nf[2] : -k[2];
+        hn.push_back((int)((int64_t)hx_ * (ny + (int64_t)geom_.nf[1] * nz)));
+      } else {
+        hn.push_back(-1);
+      }
+      ik.push_back((int)i);
+      imk.push_back((int)j);
+      for (int a = 0; a < 3; ++a) kv.push_back(k[a]);
+      vk.push_back(V[i]);
+      vm.push_back(V[j]);
+      sk.push_back(Sk[i]);
+      sm.push_back(Sk[j]);
+      double ps = psi_host_[0][std::abs(k[0])];
+      if (d_ >= 2) ps *= psi_host_[1][std::abs(k[1])];
+      if (d_ >= 3) ps *= psi_host_[2][std::abs(k[2])];
+      ip.push_back(1.0 / ps);
+    }
+    npairs_ = (int)h.size();
+    const size_t np = std::max<size_t>(npairs_, 1);
+    auto up_i = [&](DeviceBuffer<int>& b, std::vector<int>& v, size_t cnt) {
+      v.resize(cnt, 0);
+      b.allocate(cnt);
+      b.upload(v.data(), cnt);
+    };
+    auto up_d = [&](DeviceBuffer<double>& b, std::vector<double>& v) {
+      v.resize(np, 0.0);
+      b.allocate(np);
+      b.upload(v.data(), np);
+    };
+    up_i(p_h_, h, np);
+    up_i(p_hn_, hn, np);
+    up_i(p_ik_, ik, np);
+    up_i(p_imk_, imk, np);
+    up_i(p_kv_, kv, 3 * np);
+    up_d(p_vk_, vk);
+    up_d(p_vm_, vm);
+    up_d(p_sk_, sk);
+    up_d(p_sm_, sm);
+    up_d(p_ip_, ip);
+    pd_ = PairDev{p_h_.get(), p_hn_.get(), p_ik_.get(), p_imk_.get(), p_kv_.get(), p_vk_.get(),
+                  p_vm_.get(), p_sk_.get(), p_sm_.get(), p_ip_.get()};
+    nblk_ = std::max(1, (npairs_ + 255) / 256);
+    partial_.allocate(nblk_);
+    e_dev_.allocate(1);
+  }
+
+  int64_t N_;
+  int d_;
+  bool plain_;
+  int kp_ = 0, nm_ = 1;
+  int64_t M_ = 0;
+  double L_[3] = {1, 1, 1};
+  EsParams es_;
+  GridGeom geom_;
+  SortedPoints<T> pts_;
+  Spreader<T> spreader_;
+  DeviceBuffer<T> c_;
+  DeviceBuffer<T> grid_;
+  int hx_ = 1;
+  int64_t nspec_ = 0;
+  DeviceBuffer<Cplx<T>> S_;
+  DeviceBuffer<Cplx<T>> X_[3];
+  int nX_ = 1;
+  FftPlan r2c_, c2r_;
+  std::vector<double> psi_host_[3];
+  DeviceBuffer<double> psi_[3];
+  DeviceBuffer<int> p_h_, p_hn_, p_ik_, p_imk_, p_kv_;
+  DeviceBuffer<double> p_vk_, p_vm_, p_sk_, p_sm_, p_ip_;
+  PairDev pd_{};
+  int npairs_ = 0, nblk_ = 1;
+  DeviceBuffer<double> partial_, e_dev_;
+  HostScalar host_;
+  CoordMap map_e_, map_g_;
+  double dscale_[3] = {0, 0, 0};
+};
+
+}  // namespace
+
+std::unique_ptr<Potential> make_nuwu(const double* radii, int64_t n, int d, double K, const double* Sk,
+                                     const double* V, int64_t m, const double* L, double eps, int precision,
+                                     double sigma) {
+  if (precision == 1) return std::make_unique<NUwU<float>>(radii, n, d, K, Sk, V, m, L, eps, precision, sigma);
+  if (precision == 0 || precision == 2)
+    return std::make_unique<NUwU<double>>(radii, n, d, K, Sk, V, m, L, eps, precision, sigma);
+  throw InvalidArgument("unknown precision " + std::to_string(precision));
+}
+
+}  // namespace fresco_hip

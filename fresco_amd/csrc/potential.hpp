@@ -1,0 +1,65 @@
+// Device-resident potentials: the C++ counterpart of fresco::BasePotential
+// (source/fresco/base_potential.hpp:15-156) with x / grad in device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "prof.hpp"
+
+namespace fresco_hip {
+
+class Potential {
+ public:
+  virtual ~Potential() = default;
+  virtual int64_t ndof() const = 0;
+  virtual int64_t nmodes() const { return 0; }
+  // base_potential.hpp:28-31
+  virtual double energy(const double* x, hipStream_t s) = 0;
+  // base_potential.hpp:38-43; grad (device, ndof) is overwritten
+  virtual double energy_gradient(const double* x, double* grad, hipStream_t s) = 0;
+  // rho_k of the last evaluation, complex interleaved float64 (device)
+  virtual void get_rho(double* rho, hipStream_t s) {
+    (void)rho;
+    (void)s;
+    throw RuntimeFailure("this potential does not expose rho");
+  }
+  virtual void plan_info(int64_t* nf, int* w, double* beta, int* tile) const {
+    for (int a = 0; a < 3; ++a) {
+      nf[a] = 0;
+      tile[a] = 0;
+    }
+    *w = 0;
+    *beta = 0;
+  }
+};
+
+// Pinned host scalar used to return energies from the device.
+class HostScalar {
+ public:
+  HostScalar() { FRESCO_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p_), sizeof(double))); }
+  ~HostScalar() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  HostScalar(const HostScalar&) = delete;
+  HostScalar& operator=(const HostScalar&) = delete;
+  double fetch(const double* dev, hipStream_t s) {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(p_, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+    if (prof_enabled()) prof_collect();
+    return *p_;
+  }
+
+ private:
+  double* p_ = nullptr;
+};
+
+std::unique_ptr<Potential> make_nuwu(const double* radii, int64_t n, int d, double K, const double* Sk,
+                                     const double* V, int64_t m, const double* L, double eps, int precision,
+                                     double sigma);
+
+}  // namespace fresco_hip

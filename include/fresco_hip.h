@@ -1,0 +1,127 @@
+/*
+ * fresco_hip.h -- C ABI of the MI355X (gfx950) structure-factor energy/gradient
+ * library libfresco_hip.so.
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = OK, otherwise a FRESCO_E_* code;
+ *     fresco_last_error() returns a thread-local message for the last failure.
+ *   - Constructor arrays (radii, Sk, V, L, k vectors) are HOST pointers; the
+ *     handle copies what it needs to device memory and owns it.
+ *   - Evaluation arrays (x, grad) are DEVICE pointers to float64, owned by the
+ *     caller.  `stream` is a hipStream_t (NULL = default stream).  The energy is
+ *     returned through a HOST double*; the call synchronises the stream once to
+ *     deliver it.
+ *   - A handle is not re-entrant; distinct handles are independent.
+ *
+ * Every entry point cites the reference interface it replaces.
+ */
+#ifndef FRESCO_HIP_H
+#define FRESCO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FRESCO_OK 0
+#define FRESCO_E_INVALID 1   /* bad argument: maps to ValueError (std::invalid_argument) */
+#define FRESCO_E_RUNTIME 2   /* runtime failure: maps to RuntimeError (std::runtime_error) */
+#define FRESCO_E_HIP 3       /* HIP / rocFFT failure: maps to RuntimeError */
+
+/* precision / algorithm of a structure-factor potential */
+#define FRESCO_PREC_FP64_PLAIN 0   /* FP64 grids + FFTs, d separate type-2 transforms (reference structure) */
+#define FRESCO_PREC_FP32_ADJOINT 1 /* FP32 grids + FFTs, one inverse transform + kernel-derivative interpolation */
+#define FRESCO_PREC_FP64_ADJOINT 2 /* FP64 grids + FFTs, one inverse transform + kernel-derivative interpolation */
+
+typedef struct fresco_pot fresco_pot; /* opaque potential handle */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fresco_last_error(void);
+
+/* Library version string. */
+const char* fresco_version(void);
+
+/*
+ * NUwU: non-uniform points, uniform k grid.
+ * Replaces fresco::NUwU::NUwU (source/fresco/nuwu.hpp:35-52) as constructed by
+ * fresco/potentials/nuwu.pyx:10-23.
+ *   radii[n]      point radii (weights c_i = (pi r^2)^d or ((4/3) pi r^3)^d, normalised to mean 1)
+ *   d             dimension 1..3 (len(L))
+ *   K             stored, unused by the arithmetic (as in the reference)
+ *   Sk[m], V[m]   target S0(k) and weights on the (2K'+1)^d mode grid, flat order with x fastest
+ *   L[d]          box
+ *   eps           NUFFT tolerance
+ *   precision     FRESCO_PREC_*
+ *   sigma         fine-grid upsampling factor (0 = default 2.0)
+ * m must equal (2K'+1)^d exactly (the reference silently truncates other sizes).
+ */
+int fresco_nuwu_create(const double* radii, int64_t n, int d, double K,
+                       const double* Sk, const double* V, int64_t m,
+                       const double* L, double eps, int precision, double sigma,
+                       fresco_pot** out);
+
+/*
+ * Energy of coordinates x[n*d] (device, interleaved x0,y0,z0,x1,...).
+ * Replaces fresco::BasePotential::get_energy (base_potential.hpp:28-31), i.e.
+ * NUwU::get_energy (nuwu.hpp:152-208) and its siblings.
+ */
+int fresco_pot_energy(fresco_pot* pot, const double* x, double* energy, void* stream);
+
+/*
+ * Energy and gradient; grad[n*d] (device) is overwritten.
+ * Replaces fresco::BasePotential::get_energy_gradient (base_potential.hpp:38-43),
+ * i.e. NUwU::get_energy_gradient (nuwu.hpp:210-317) and its siblings; called by the
+ * optimizers at optimizer.hpp:208-215.
+ */
+int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad,
+                               double* energy, void* stream);
+
+/*
+ * Copy the last evaluation's rho_k (complex, interleaved re/im, reference flat mode
+ * order, length = number of modes) into rho[2*m] (device, float64).  The S(k) the
+ * reference would compute from it is |rho_k|^2 / N (nuwu.hpp:262-264).
+ * No reference counterpart: exposes an intermediate of nuwu.hpp:258 for the
+ * final-S(k) report and the autograd backward.
+ */
+int fresco_pot_get_rho(fresco_pot* pot, double* rho, void* stream);
+
+/* Number of degrees of freedom the potential expects (n*d for points, prod(N) for grids). */
+int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof);
+
+/* Number of modes / k vectors of the potential. */
+int fresco_pot_nmodes(fresco_pot* pot, int64_t* nmodes);
+
+/*
+ * Introspection of the NUFFT set-up: fine grid sizes nf[3], kernel width w,
+ * kernel beta and the tile sizes used by the spreader.
+ */
+int fresco_pot_plan_info(fresco_pot* pot, int64_t* nf, int* w, double* beta, int* tile);
+
+/* Release a handle and all device memory / FFT plans it owns (the reference never frees its plans). */
+int fresco_pot_destroy(fresco_pot* pot);
+
+/*
+ * Stage profiler (HIP events recorded on the evaluation's stream around each
+ * stage: 0 bin_sort, 1 spread, 2 fft_fwd, 3 zero_fill, 4 loss, 5 fft_inv,
+ * 6 interp).  fresco_prof_get returns the stage name, the summed duration in
+ * milliseconds and the number of timed launches since the last reset.
+ * No reference counterpart (the reference has no timers; its only progress
+ * output is the optimizer's iprint line, lbfgs.cpp:56-62).
+ */
+int fresco_prof_enable(int on);
+int fresco_prof_reset(void);
+int fresco_prof_get(int stage, const char** name, double* total_ms, int64_t* count);
+
+/*
+ * Failure injection for tests: the next `count` device allocations fail with
+ * FRESCO_E_HIP.  No reference counterpart (the reference has no fault hooks).
+ */
+int fresco_set_fault(int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRESCO_HIP_H */

@@ -1,0 +1,174 @@
+"""GPU parity of NUwU (libfresco_hip through the C ABI) against the oracle.
+
+Tolerances (relative l2 unless stated):
+  fp64 / fp64-adjoint at eps=1e-12 : energy 1e-9, gradient 1e-9, rho 1e-9
+  fp32 (adjoint)      at eps=1e-6  : energy 1e-5, gradient 1e-5, rho 1e-5
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import sf_direct as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-9, "fp64-adjoint": 1e-9, "fp32": 1e-5}
+EPS = {"fp64": 1e-12, "fp64-adjoint": 1e-12, "fp32": 1e-6}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def make_case(d, n, N, seed, spread_coords=True):
+    rng = np.random.default_rng(seed)
+    M = n ** d
+    radii = rng.uniform(0.5, 1.5, N)
+    Sk = rng.uniform(0, 2, M)
+    Sk[::3] = 0.0
+    V = rng.uniform(0, 1, M)
+    V[::5] = 0.0
+    lo, hi = (-0.3, 1.3) if spread_coords else (0.0, 1.0)
+    p = rng.uniform(lo, hi, N * d)
+    return radii, Sk, V, p
+
+
+@pytest.fixture(scope="module")
+def NUwU():
+    from fresco_amd.potentials import NUwU as cls
+
+    return cls
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64-adjoint", "fp32"])
+@pytest.mark.parametrize("d,n,N", [(1, 33, 200), (2, 17, 400), (3, 9, 300)])
+def test_parity_vs_direct_sum(NUwU, prec, d, n, N):
+    radii, Sk, V, p = make_case(d, n, N, seed=10 * d + n)
+    L = np.ones(d)
+    ref = O.NUwU(radii, 0.0, Sk, V, L)
+    E0, g0 = ref.get_energy_gradient(p)
+    pot = NUwU(radii, 0.0, Sk, V, L, EPS[prec], precision=prec)
+    E, g = pot.get_energy_gradient(p)
+    print("case", d, n, N, prec, pot.plan_info(), flush=True)
+    assert abs(E - E0) / abs(E0) < TOL[prec]
+    assert rel(g, g0) < TOL[prec]
+    assert rel(pot.get_rho(), ref.rho(p)) < TOL[prec]
+    Ee = pot.get_energy(p)
+    assert abs(Ee - ref.get_energy(p)) / abs(E0) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_box_not_one(NUwU, prec):
+    # the reference map (p - round(p/L)) * 2pi/L is kept for L != 1 (nuwu.hpp:219)
+    d = 2
+    radii, Sk, V, p = make_case(d, 13, 150, seed=5)
+    L = np.array([2.0, 0.5])
+    ref = O.NUwU(radii, 0.0, Sk, V, L)
+    E0, g0 = ref.get_energy_gradient(p)
+    pot = NUwU(radii, 0.0, Sk, V, L, EPS[prec], precision=prec)
+    E, g = pot.get_energy_gradient(p)
+    assert abs(E - E0) / abs(E0) < TOL[prec]
+    assert rel(g, g0) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64-adjoint", "fp32"])
+def test_square_lattice_closed_form(NUwU, prec):
+    # points exactly on a lattice (and on fine-grid cells): E = count * N^2
+    n = 8
+    g = (np.arange(n) + 0.5) / n
+    p = np.stack(np.meshgrid(g, g, indexing="ij"), axis=-1).ravel()
+    N = n * n
+    kp = 12
+    M = (2 * kp + 1) ** 2
+    pot = NUwU(np.ones(N), 0, np.zeros(M), np.ones(M), np.ones(2), EPS[prec], precision=prec)
+    count = sum(1 for a in range(-kp, kp + 1) for b in range(-kp, kp + 1) if a % n == 0 and b % n == 0)
+    E, grad = pot.get_energy_gradient(p)
+    assert abs(E - count * N ** 2) / (count * N ** 2) < TOL[prec]
+    # a perfect lattice is a stationary point
+    assert np.abs(grad).max() < 1e-4 * N if prec == "fp32" else np.abs(grad).max() < 1e-6
+
+
+def test_deterministic(NUwU):
+    radii, Sk, V, p = make_case(3, 9, 500, seed=11)
+    for prec in ("fp32", "fp64"):
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), EPS[prec], precision=prec)
+        E1, g1 = pot.get_energy_gradient(p)
+        E2, g2 = pot.get_energy_gradient(p)
+        assert E1 == E2
+        assert np.array_equal(g1, g2)
+
+
+def test_torch_tensor_in_device_tensor_out(NUwU):
+    import torch
+
+    radii, Sk, V, p = make_case(2, 11, 100, seed=12)
+    pot = NUwU(radii, 0.0, Sk, V, np.ones(2), 1e-10, precision="fp64")
+    xt = torch.tensor(p, device="cuda")
+    E, g = pot.get_energy_gradient(xt)
+    assert isinstance(g, torch.Tensor) and g.is_cuda
+    E2, g2 = pot.get_energy_gradient(p)
+    assert E == E2
+    assert np.array_equal(g.cpu().numpy(), g2)
+
+
+def test_invalid_sizes_and_faults(NUwU):
+    from fresco_amd import _lib
+
+    with pytest.raises(ValueError):
+        NUwU(np.ones(10), 0, np.zeros(64), np.ones(64), np.ones(2))  # 8^2 is not (2K+1)^2
+    with pytest.raises(ValueError):
+        NUwU(np.ones(10), 0, np.zeros(9), np.ones(9), np.ones(2), precision="fp16")
+    pot = NUwU(np.ones(10), 0, np.zeros(9), np.ones(9), np.ones(2))
+    with pytest.raises(ValueError):
+        pot.get_energy(np.zeros(7))
+    _lib.check(_lib.load().fresco_set_fault(1))
+    with pytest.raises(_lib.FrescoError):
+        NUwU(np.ones(10), 0, np.zeros(9), np.ones(9), np.ones(2))
+    _lib.check(_lib.load().fresco_set_fault(0))
+    NUwU(np.ones(10), 0, np.zeros(9), np.ones(9), np.ones(2))
+
+
+def test_fd_directional_derivative_fp64(NUwU):
+    # mid-size 3D case, fp64: (E(p + h v) - E(p - h v)) / 2h == grad . v * 2pi
+    rng = np.random.default_rng(13)
+    N, n, d = 4000, 33, 3
+    M = n ** d
+    V = (np.linalg.norm(O.mode_grid(16, 3), axis=1) <= 12).astype(float)
+    V[M // 2] = 0
+    pot = NUwU(np.ones(N), 0, np.zeros(M), V, np.ones(d), 1e-12, precision="fp64")
+    p = rng.uniform(0, 1, N * d)
+    v = rng.normal(size=N * d)
+    h = 1e-6
+    E, g = pot.get_energy_gradient(p)
+    fd = (pot.get_energy(p + h * v) - pot.get_energy(p - h * v)) / (2 * h)
+    assert abs(fd - 2 * math.pi * np.dot(g, v)) / abs(fd) < 1e-6
+
+
+def test_c2_size_fp32_vs_fp64_and_invariants(NUwU):
+    """Stealthy |k| <= 24 on 129^3 modes, N = 2^20: FP32-adjoint vs FP64 plain
+    (energy and gradient within 1e-5), Sum(grad) = 0 and translation invariance
+    of the energy."""
+    import torch
+
+    rng = np.random.default_rng(0)
+    N, n, d = 1 << 20, 129, 3
+    k = O.mode_grid(64, 3)
+    kk = np.linalg.norm(k, axis=1)
+    V = ((kk > 0) & (kk <= 24)).astype(float)
+    Sk = np.zeros_like(V)
+    p = rng.uniform(0, 1, N * d)
+    p64 = NUwU(np.ones(N), 64, Sk, V, np.ones(d), 1e-9, precision="fp64")
+    p32 = NUwU(np.ones(N), 64, Sk, V, np.ones(d), 1e-6, precision="fp32")
+    xt = torch.tensor(p, device="cuda")
+    E64, g64 = p64.get_energy_gradient(xt)
+    E32, g32 = p32.get_energy_gradient(xt)
+    print("C2 energies", E64, E32, flush=True)
+    assert abs(E32 - E64) / abs(E64) < 1e-5
+    r = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+    print("C2 gradient rel err fp32 vs fp64", r, flush=True)
+    assert r < 1e-5
+    gs = g64.reshape(-1, 3).sum(0).abs().max().item()
+    assert gs < 1e-8 * float(g64.abs().max()) * N ** 0.5
+    Es = p64.get_energy(xt + 0.123456)
+    assert abs(Es - E64) / abs(E64) < 1e-9
