@@ -53,6 +53,31 @@ SIGNATURES = {
         [ctypes.c_void_p, c_int64_p, c_int_p, c_double_p, c_int_p],
     ),
     "fresco_pot_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fresco_combined_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, c_void_pp]),
+    "fresco_callback_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
+    "fresco_lbfgs_create": (
+        ctypes.c_int,
+        [ctypes.c_void_p, c_double_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, c_void_pp],
+    ),
+    "fresco_fire_create": (
+        ctypes.c_int,
+        [ctypes.c_void_p, c_double_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int, c_void_pp],
+    ),
+    "fresco_opt_set_param": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    "fresco_opt_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fresco_opt_one_iteration": (ctypes.c_int, [ctypes.c_void_p]),
+    "fresco_opt_reset": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int64]),
+    "fresco_opt_set_func_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p, ctypes.c_int64]),
+    "fresco_opt_stop_criterion": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    "fresco_opt_get_state": (
+        ctypes.c_int,
+        [ctypes.c_void_p, c_double_p, c_double_p, c_int_p, c_int_p, c_int_p, c_double_p],
+    ),
+    "fresco_opt_get_x": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    "fresco_opt_get_g": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),
     "fresco_prof_reset": (ctypes.c_int, []),
     "fresco_prof_get": (
@@ -62,6 +87,10 @@ SIGNATURES = {
 }
 
 N_STAGES = 7
+
+OPT_TOL, OPT_MAXSTEP, OPT_MAX_ITER, OPT_IPRINT, OPT_VERBOSITY, OPT_H0, OPT_MAX_F_RISE, OPT_USE_RELATIVE_F = range(8)
+
+EG_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, c_double_p, c_double_p, c_double_p, ctypes.c_void_p)
 
 
 def prof_enable(on=True):

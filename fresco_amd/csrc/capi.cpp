@@ -11,11 +11,17 @@
 
 #include "../../include/fresco_hip.h"
 #include "common.hpp"
+#include "optim.hpp"
 #include "potential.hpp"
 #include "prof.hpp"
 
 struct fresco_pot {
-  std::unique_ptr<fresco_hip::Potential> impl;
+  std::shared_ptr<fresco_hip::Potential> impl;
+};
+
+struct fresco_opt {
+  std::unique_ptr<fresco_hip::GradientOptimizer> impl;
+  fresco_hip::LBFGS* lbfgs = nullptr;
 };
 
 namespace fresco_hip {
@@ -65,6 +71,11 @@ fresco_hip::Potential* checked(fresco_pot* p) {
   return p->impl.get();
 }
 
+fresco_hip::GradientOptimizer* checked(fresco_opt* o) {
+  if (!o || !o->impl) throw fresco_hip::InvalidArgument("null optimizer handle");
+  return o->impl.get();
+}
+
 }  // namespace
 
 extern "C" {
@@ -88,6 +99,149 @@ int fresco_nuwu_create(const double* radii, int64_t n, int d, double K, const do
     h->impl = fresco_hip::make_nuwu(radii, n, d, K, Sk, V, m, L, eps, precision, sigma);
     *out = h.release();
   });
+}
+
+int fresco_combined_create(fresco_pot* const* pots, int count, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!pots || count <= 0) throw fresco_hip::InvalidArgument("no potentials");
+    std::vector<std::shared_ptr<fresco_hip::Potential>> v;
+    for (int i = 0; i < count; ++i) {
+      checked(pots[i]);
+      v.push_back(pots[i]->impl);
+    }
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_combined(std::move(v));
+    *out = h.release();
+  });
+}
+
+int fresco_callback_create(int64_t ndof, fresco_eg_callback cb, void* user, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_callback(ndof, cb, user);
+    *out = h.release();
+  });
+}
+
+int fresco_lbfgs_create(fresco_pot* pot, const double* x0, int64_t n, double tol, int M, fresco_opt** out) {
+  return guarded([&]() {
+    if (!out || !x0) throw fresco_hip::InvalidArgument("null pointer");
+    *out = nullptr;
+    checked(pot);
+    auto h = std::make_unique<fresco_opt>();
+    auto* o = new fresco_hip::LBFGS(pot->impl, x0, n, tol, M);
+    h->impl.reset(o);
+    h->lbfgs = o;
+    *out = h.release();
+  });
+}
+
+int fresco_fire_create(fresco_pot* pot, const double* x0, int64_t n, double dtstart, double dtmax, double maxstep,
+                       int64_t Nmin, double finc, double fdec, double fa, double astart, double tol, int stepback,
+                       fresco_opt** out) {
+  return guarded([&]() {
+    if (!out || !x0) throw fresco_hip::InvalidArgument("null pointer");
+    *out = nullptr;
+    checked(pot);
+    auto h = std::make_unique<fresco_opt>();
+    h->impl.reset(new fresco_hip::ModifiedFire(pot->impl, x0, n, dtstart, dtmax, maxstep, Nmin, finc, fdec, fa,
+                                               astart, tol, stepback != 0));
+    *out = h.release();
+  });
+}
+
+int fresco_opt_set_param(fresco_opt* opt, int param, double value) {
+  return guarded([&]() {
+    auto* o = checked(opt);
+    switch (param) {
+      case FRESCO_OPT_TOL: o->set_tol(value); break;
+      case FRESCO_OPT_MAXSTEP: o->set_maxstep(value); break;
+      case FRESCO_OPT_MAX_ITER: o->set_max_iter((int)value); break;
+      case FRESCO_OPT_IPRINT: o->set_iprint((int)value); break;
+      case FRESCO_OPT_VERBOSITY: o->set_verbosity((int)value); break;
+      case FRESCO_OPT_H0:
+      case FRESCO_OPT_MAX_F_RISE:
+      case FRESCO_OPT_USE_RELATIVE_F:
+        if (!opt->lbfgs) throw fresco_hip::InvalidArgument("parameter only applies to LBFGS");
+        if (param == FRESCO_OPT_H0) opt->lbfgs->set_H0(value);
+        if (param == FRESCO_OPT_MAX_F_RISE) opt->lbfgs->set_max_f_rise(value);
+        if (param == FRESCO_OPT_USE_RELATIVE_F) opt->lbfgs->set_use_relative_f((int)value);
+        break;
+      default: throw fresco_hip::InvalidArgument("unknown optimizer parameter " + std::to_string(param));
+    }
+  });
+}
+
+int fresco_opt_run(fresco_opt* opt, int niter) {
+  return guarded([&]() {
+    auto* o = checked(opt);
+    if (niter < 0)
+      o->run();
+    else
+      o->run(niter);
+  });
+}
+
+int fresco_opt_one_iteration(fresco_opt* opt) {
+  return guarded([&]() { checked(opt)->one_iteration(); });
+}
+
+int fresco_opt_reset(fresco_opt* opt, const double* x0, int64_t n) {
+  return guarded([&]() {
+    auto* o = checked(opt);
+    if (!x0) throw fresco_hip::InvalidArgument("null pointer");
+    if (n != o->size())
+      throw fresco_hip::InvalidArgument(
+          "The number of degrees of freedom (x0.size()) cannot change when calling reset()");
+    o->reset(x0);
+  });
+}
+
+int fresco_opt_set_func_gradient(fresco_opt* opt, double f, const double* grad, int64_t n) {
+  return guarded([&]() {
+    auto* o = checked(opt);
+    if (!grad) throw fresco_hip::InvalidArgument("null pointer");
+    if (n != o->size()) throw fresco_hip::InvalidArgument("the gradient has the wrong size");
+    o->set_func_gradient(f, grad);
+  });
+}
+
+int fresco_opt_stop_criterion(fresco_opt* opt, int* satisfied) {
+  return guarded([&]() { *satisfied = checked(opt)->stop_criterion_satisfied() ? 1 : 0; });
+}
+
+int fresco_opt_get_state(fresco_opt* opt, double* f, double* rms, int* niter, int* nfev, int* maxiter, double* H0) {
+  return guarded([&]() {
+    auto* o = checked(opt);
+    if (f) *f = o->get_f();
+    if (rms) *rms = o->get_rms();
+    if (niter) *niter = o->get_niter();
+    if (nfev) *nfev = o->get_nfev();
+    if (maxiter) *maxiter = o->get_maxiter();
+    if (H0) *H0 = o->get_H0();
+  });
+}
+
+int fresco_opt_get_x(fresco_opt* opt, double* x) {
+  return guarded([&]() {
+    if (!x) throw fresco_hip::InvalidArgument("null pointer");
+    checked(opt)->get_x(x);
+  });
+}
+
+int fresco_opt_get_g(fresco_opt* opt, double* g) {
+  return guarded([&]() {
+    if (!g) throw fresco_hip::InvalidArgument("null pointer");
+    checked(opt)->get_g(g);
+  });
+}
+
+int fresco_opt_destroy(fresco_opt* opt) {
+  return guarded([&]() { delete opt; });
 }
 
 int fresco_pot_energy(fresco_pot* pot, const double* x, double* energy, void* stream) {

@@ -58,6 +58,13 @@ class HostScalar {
   double* p_ = nullptr;
 };
 
+// Host callback: x (host, ndof) -> *energy and, if grad != NULL, grad (host).
+// Returns 0 on success.
+typedef int (*HostEgCallback)(const double* x, double* grad, double* energy, void* user);
+
+std::shared_ptr<Potential> make_combined(std::vector<std::shared_ptr<Potential>> pots);
+std::shared_ptr<Potential> make_callback(int64_t n, HostEgCallback cb, void* user);
+
 std::unique_ptr<Potential> make_nuwu(const double* radii, int64_t n, int d, double K, const double* Sk,
                                      const double* V, int64_t m, const double* L, double eps, int precision,
                                      double sigma);

@@ -36,6 +36,21 @@ extern "C" {
 #define FRESCO_PREC_FP64_ADJOINT 2 /* FP64 grids + FFTs, one inverse transform + kernel-derivative interpolation */
 
 typedef struct fresco_pot fresco_pot; /* opaque potential handle */
+typedef struct fresco_opt fresco_opt; /* opaque optimizer handle */
+
+/* Host energy(+gradient) callback: x[ndof] (host) -> *energy and, when grad is
+ * not NULL, grad[ndof] (host).  Returns 0 on success. */
+typedef int (*fresco_eg_callback)(const double* x, double* grad, double* energy, void* user);
+
+/* optimizer parameters for fresco_opt_set_param */
+#define FRESCO_OPT_TOL 0            /* optimizer.hpp:173 set_tol */
+#define FRESCO_OPT_MAXSTEP 1        /* optimizer.hpp:174 set_maxstep */
+#define FRESCO_OPT_MAX_ITER 2       /* optimizer.hpp:175 set_max_iter */
+#define FRESCO_OPT_IPRINT 3         /* optimizer.hpp:176 set_iprint */
+#define FRESCO_OPT_VERBOSITY 4      /* optimizer.hpp:177 set_verbosity */
+#define FRESCO_OPT_H0 5             /* lbfgs.hpp:70-76 set_H0 */
+#define FRESCO_OPT_MAX_F_RISE 6     /* lbfgs.hpp:77 set_max_f_rise */
+#define FRESCO_OPT_USE_RELATIVE_F 7 /* lbfgs.hpp:79-82 set_use_relative_f */
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* fresco_last_error(void);
@@ -101,6 +116,69 @@ int fresco_pot_plan_info(fresco_pot* pot, int64_t* nf, int* w, double* beta, int
 
 /* Release a handle and all device memory / FFT plans it owns (the reference never frees its plans). */
 int fresco_pot_destroy(fresco_pot* pot);
+
+/*
+ * Sum of potentials over the same coordinates.
+ * Replaces fresco::CombinedPotential (source/fresco/combine_potentials.hpp:16-59),
+ * fresco/potentials/combine_potentials.pyx:10-30.  The children stay alive while
+ * the combination exists (shared ownership); destroying a child handle is allowed.
+ */
+int fresco_combined_create(fresco_pot* const* pots, int count, fresco_pot** out);
+
+/*
+ * Potential implemented by a host callback (ndof coordinates).
+ * Replaces the PythonPotential route (source/fresco/python_potential_wrapper.hpp:43-184,
+ * fresco/potentials/_pythonpotential.pyx:58-65).  x is copied device->host before
+ * and grad host->device after each call.
+ */
+int fresco_callback_create(int64_t ndof, fresco_eg_callback cb, void* user, fresco_pot** out);
+
+/*
+ * LBFGS with backtracking line search on the device.
+ * Replaces fresco::LBFGS::LBFGS (source/lbfgs.cpp:9-31) as built by
+ * fresco/optimize/_lbfgs_cpp.pyx:43-70.  x0[n] is a host array; other settings via
+ * fresco_opt_set_param.
+ */
+int fresco_lbfgs_create(fresco_pot* pot, const double* x0, int64_t n, double tol, int M, fresco_opt** out);
+
+/*
+ * Modified FIRE on the device.
+ * Replaces fresco::MODIFIED_FIRE::MODIFIED_FIRE (source/modified_fire.cpp:43-68) as
+ * built by fresco/optimize/_modified_fire_cpp.pyx:18-41.
+ */
+int fresco_fire_create(fresco_pot* pot, const double* x0, int64_t n, double dtstart, double dtmax,
+                       double maxstep, int64_t Nmin, double finc, double fdec, double fa, double astart,
+                       double tol, int stepback, fresco_opt** out);
+
+/* Setters of optimizer.hpp:173-177 and lbfgs.hpp:70-82 (FRESCO_OPT_*). */
+int fresco_opt_set_param(fresco_opt* opt, int param, double value);
+
+/* GradientOptimizer::run(niter) (optimizer.hpp:118-133); niter < 0 runs run()
+ * (optimizer.hpp:139-142, up to max_iter). */
+int fresco_opt_run(fresco_opt* opt, int niter);
+
+/* one_iteration (lbfgs.cpp:36-64, modified_fire.hpp:160-230). */
+int fresco_opt_one_iteration(fresco_opt* opt);
+
+/* reset(x0) (lbfgs.cpp:247-257, modified_fire.hpp:84-112); x0[n] host. */
+int fresco_opt_reset(fresco_opt* opt, const double* x0, int64_t n);
+
+/* set_func_gradient (optimizer.hpp:148-165, modified_fire.cpp:94-118); grad[n] host. */
+int fresco_opt_set_func_gradient(fresco_opt* opt, double f, const double* grad, int64_t n);
+
+/* stop_criterion_satisfied (optimizer.hpp:195-201). */
+int fresco_opt_stop_criterion(fresco_opt* opt, int* satisfied);
+
+/* get_f / get_rms / get_niter / get_nfev / get_maxiter / get_H0 (optimizer.hpp:181-189,
+ * lbfgs.hpp:86); any pointer may be NULL. */
+int fresco_opt_get_state(fresco_opt* opt, double* f, double* rms, int* niter, int* nfev, int* maxiter,
+                         double* H0);
+
+/* get_x / get_g (optimizer.hpp:181-182) into host arrays of size n. */
+int fresco_opt_get_x(fresco_opt* opt, double* x);
+int fresco_opt_get_g(fresco_opt* opt, double* g);
+
+int fresco_opt_destroy(fresco_opt* opt);
 
 /*
  * Stage profiler (HIP events recorded on the evaluation's stream around each
