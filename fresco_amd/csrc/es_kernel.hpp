@@ -27,7 +27,10 @@ struct EsParams {
   double beta = 16.1;
 };
 
-EsParams es_params(double eps, double sigma);
+// extra_w widens the kernel beyond the eps rule (the kernel-derivative
+// interpolation of the adjoint path amplifies the aliasing error by about the
+// ratio of the fine-grid Nyquist to the mode band, one more digit in 2D/3D).
+EsParams es_params(double eps, double sigma, int extra_w = 0);
 
 // Psi(k) for k = 0..kmax on a periodic grid of n cells (float64).
 std::vector<double> es_deconv(const EsParams& p, int n, int kmax);

@@ -17,21 +17,21 @@ namespace fresco_hip {
 // ES kernel parameters and deconvolution factors (host, float64)
 // ---------------------------------------------------------------------------
 
-EsParams es_params(double eps, double sigma) {
+EsParams es_params(double eps, double sigma, int extra_w) {
   if (!(eps > 0)) throw InvalidArgument("eps must be > 0");
   if (!(sigma > 1.0)) throw InvalidArgument("upsampling factor sigma must be > 1");
   EsParams p;
   int w;
   double bw;
   if (sigma == 2.0) {
-    w = (int)std::ceil(-std::log10(eps / 10.0));
+    w = (int)std::ceil(-std::log10(eps / 10.0)) + extra_w;
     w = std::max(2, w);
     bw = 2.30;
     if (w == 2) bw = 2.20;
     if (w == 3) bw = 2.26;
     if (w == 4) bw = 2.38;
   } else {
-    w = (int)std::ceil(-std::log(eps) / (M_PI * std::sqrt(1.0 - 1.0 / sigma)));
+    w = (int)std::ceil(-std::log(eps) / (M_PI * std::sqrt(1.0 - 1.0 / sigma))) + extra_w;
     w = std::max(2, w);
     bw = 0.97 * M_PI * (1.0 - 1.0 / (2.0 * sigma));
   }
@@ -111,54 +111,80 @@ static int ceil_log2(long long v) {
   return std::max(b, 1);
 }
 
+int lds_row_stride(int n, int elem_bytes, int w) {
+  // Row stride (elements) for the (ja, jb) lane pattern: consecutive kernel
+  // rows of one half-wavefront should land on disjoint LDS banks.
+  const int wpe = elem_bytes / 4;
+  const int wmax = w <= 8 ? 8 : 16;
+  const int target = (wmax * wpe) % 32;
+  int words = n * wpe;
+  if (target != 0) words += ((target - words % 32) + 32) % 32;
+  return words / wpe;
+}
+
+size_t spread_lds_bytes(const GridGeom& g, int elem_bytes, int w) {
+  int sx = g.d >= 2 ? lds_row_stride(g.tile[0], elem_bytes, w) : g.tile[0];
+  return (size_t)elem_bytes * sx * (g.d >= 2 ? g.tile[1] : 1) * (g.d >= 3 ? g.tile[2] : 1);
+}
+
+size_t interp_lds_bytes(const GridGeom& g, int elem_bytes, int w) {
+  int rx = g.tile[0] + w - 1;
+  int ry = g.d >= 2 ? g.tile[1] + w - 1 : 1;
+  int rz = g.d >= 3 ? g.tile[2] + w - 1 : 1;
+  int srx = g.d >= 2 ? lds_row_stride(rx, elem_bytes, w) : rx;
+  return (size_t)elem_bytes * srx * ry * rz;
+}
+
 GridGeom make_geom(int d, const int* nf, int w, int elem_bytes) {
   GridGeom g;
   g.d = d;
   for (int a = 0; a < 3; ++a) g.nf[a] = a < d ? nf[a] : 1;
-  // power-of-two tiles; shrink until there are enough tiles to fill 256 CUs.
-  int t3 = 16, t2 = 64, t1 = 1024;
   for (int a = 0; a < 3; ++a) g.tile[a] = 1;
   if (d == 3) {
-    for (int a = 0; a < 3; ++a) g.tile[a] = t3;
-    auto ntiles = [&]() {
-      long long t = 1;
-      for (int a = 0; a < 3; ++a) t *= (g.nf[a] + g.tile[a] - 1) / g.tile[a];
-      return t;
-    };
-    while (ntiles() < 1024 && g.tile[2] > 4) {
-      for (int a = 2; a >= 0; --a)
-        if (g.tile[a] > 4) {
-          g.tile[a] /= 2;
-          break;
-        }
-    }
+    // the register-tile spreader's layout (16 x 16 x 8); not shrunk further
+    g.tile[0] = 16;
+    g.tile[1] = 16;
+    g.tile[2] = 8;
   } else if (d == 2) {
-    g.tile[0] = g.tile[1] = t2;
-    auto ntiles = [&]() {
-      return (long long)((g.nf[0] + g.tile[0] - 1) / g.tile[0]) * ((g.nf[1] + g.tile[1] - 1) / g.tile[1]);
-    };
-    while (ntiles() < 1024 && g.tile[1] > 8) {
-      if (g.tile[1] >= g.tile[0])
-        g.tile[1] /= 2;
-      else
-        g.tile[0] /= 2;
-    }
+    g.tile[0] = g.tile[1] = 64;
   } else {
-    g.tile[0] = t1;
-    while ((g.nf[0] + g.tile[0] - 1) / g.tile[0] < 512 && g.tile[0] > 32) g.tile[0] /= 2;
+    g.tile[0] = 1024;
   }
-  for (int a = 0; a < 3; ++a) {
-    g.tile[a] = std::min(g.tile[a], g.nf[a]);
-    g.ntile[a] = (g.nf[a] + g.tile[a] - 1) / g.tile[a];
+  auto clamp = [&]() {
+    for (int a = 0; a < d; ++a) g.tile[a] = std::max(1, std::min(g.tile[a], g.nf[a]));
+  };
+  auto ntiles = [&]() {
+    long long t = 1;
+    for (int a = 0; a < 3; ++a) t *= (g.nf[a] + g.tile[a] - 1) / g.tile[a];
+    return t;
+  };
+  auto halve_largest = [&](int minsz) {
+    int best = -1;
+    for (int a = d - 1; a >= 0; --a)
+      if (g.tile[a] > minsz && (best < 0 || g.tile[a] > g.tile[best])) best = a;
+    if (best < 0) return false;
+    g.tile[best] /= 2;
+    return true;
+  };
+  clamp();
+  if (d < 3) {
+    // LDS budget: interpolation region <= 80 KB (two workgroups per CU),
+    // spreading tile <= 40 KB; enough tiles to fill 256 CUs
+    while (interp_lds_bytes(g, elem_bytes, w) > 80 * 1024 || spread_lds_bytes(g, elem_bytes, w) > 40 * 1024)
+      if (!halve_largest(4)) break;
+    const int minsz = d == 2 ? 16 : 64;
+    while (ntiles() < 1024)
+      if (!halve_largest(minsz)) break;
+    clamp();
   }
+  for (int a = 0; a < 3; ++a) g.ntile[a] = (g.nf[a] + g.tile[a] - 1) / g.tile[a];
   g.nbins = g.ntile[0] * g.ntile[1] * g.ntile[2];
   g.key_bits = ceil_log2(g.nbins + 1);
   g.cell_bits = d == 3 ? 10 : (d == 2 ? 16 : 31);
   for (int a = 0; a < d; ++a)
     if (g.nf[a] > (1 << g.cell_bits) - 1)
       throw InvalidArgument("fine grid too large for the packed cell index (" + std::to_string(g.nf[a]) + ")");
-  (void)w;
-  (void)elem_bytes;
+  if (interp_lds_bytes(g, elem_bytes, w) > 160 * 1024) throw InvalidArgument("kernel too wide for LDS tiles");
   return g;
 }
 
@@ -299,7 +325,62 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
 }
 
 // ---------------------------------------------------------------------------
-// spreading: owner-computes tiles in LDS
+// wavefront primitives
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_read(double v, int lane) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ unsigned lane_read_u(unsigned v, int lane) {
+  return (unsigned)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+// v from lane src (any lane pattern): ds_bpermute
+__device__ __forceinline__ float lane_get(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_get(double v, int src) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b & 0xffffffffLL));
+  int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// DPP move (out-of-range lanes read 0)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wavefront; the result is valid in lane 63.
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_shr:4, row_shr:8,
+// row_bcast:15, row_bcast:31 (GFX9 DPP).
+template <typename T>
+__device__ __forceinline__ T wave_sum63(T v) {
+  v += dpp_f<0xb1>(v);
+  v += dpp_f<0x4e>(v);
+  v += dpp_f<0x114>(v);
+  v += dpp_f<0x118>(v);
+  v += dpp_f<0x142>(v);
+  v += dpp_f<0x143>(v);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// spreading: one wavefront owns one tile (LDS), owner-computes
 // ---------------------------------------------------------------------------
 
 template <typename T>
@@ -308,7 +389,6 @@ struct SpreadDev {
   int cbits;
   int w;
   int sx;        // LDS row stride (tile[0] + pad)
-  int slab;      // cells of the slowest tile dimension owned by each wave
   int maxc[3];   // candidate bins per dimension (table width)
   int cstride;   // table entries per dimension block
   EsDev<T> k;
@@ -319,30 +399,31 @@ struct Lanes {
   static constexpr int SH = WMAX == 8 ? 3 : 4;
 };
 
-template <typename T, int D>
-__device__ __forceinline__ T rec_comp(const typename Vec4<T>::type& r, int a) {
-  return a == 0 ? r.x : (a == 1 ? r.y : r.z);
-}
-
-// One workgroup owns one tile.  Candidate points (those binned in tiles whose
-// points can reach this tile) are staged 64 at a time: wave 0 compacts the
-// ones whose support touches the tile, all threads tabulate their kernel
-// values in LDS, then every wave adds the contributions falling in its slab of
-// the tile (slabs along the slowest dimension are disjoint, so LDS updates
-// need no atomics and each cell sees the points in one fixed order).
-template <typename T, int D, int WMAX, int NW>
-__global__ void __launch_bounds__(64 * NW)
+// The wavefront walks the candidate bins of its tile (the bins whose points
+// can reach the tile) in a fixed order, 64 candidates at a time; the ones
+// whose support touches the tile are processed one after another.  For each
+// point, lanes 0..D*WMAX-1 evaluate the D*w kernel values once; lane
+// (ja, jb) then takes w_x[ja] w_y[jb] c by ds_bpermute and adds it, times
+// w_z[jc] (broadcast by readlane), into LDS with a non-returning LDS add.
+// One wavefront per tile and in-order LDS instructions make the plain
+// read-modify-write safe and the summation order of every cell fixed
+// (bitwise reproducible); each fine-grid cell is written to HBM exactly once,
+// no atomics.  Used for 1D / 2D and for 3D grids too small for the
+// register-tile spreader.
+template <typename T, int D, int WMAX>
+__global__ void __launch_bounds__(64)
     k_spread(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
              const int* __restrict__ bin_start, const int* __restrict__ cand, SpreadDev<T> a,
              T* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
+  T* tileb = reinterpret_cast<T*>(smem_raw);
+  const int lane = threadIdx.x;
   const int w = a.w;
   const uint32_t cmask = (1u << a.cbits) - 1u;
+  constexpr int SH = Lanes<WMAX>::SH;
+  constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
 
-  int b = blockIdx.x;
+  const int b = blockIdx.x;
   int t[3];
   t[0] = b % a.ntile[0];
   t[1] = (b / a.ntile[0]) % a.ntile[1];
@@ -354,142 +435,102 @@ __global__ void __launch_bounds__(64 * NW)
     tl[q] = min(a.tile[q], a.nf[q] - t0[q]);
   }
   const int ty = a.tile[1];
+  const int zstride = ty * a.sx;
   const int tile_elems = a.sx * (D >= 2 ? a.tile[1] : 1) * (D >= 3 ? a.tile[2] : 1);
-  T* tileb = reinterpret_cast<T*>(smem_raw);
-  T* kv = tileb + tile_elems;                  // [64][D][WMAX]
-  T* pcw = kv + 64 * D * WMAX;                 // [64]
-  uint32_t* ps = reinterpret_cast<uint32_t*>(pcw + 64);  // [64]
-  __shared__ int s_np;
+  for (int e = lane; e < tile_elems; e += 64) tileb[e] = T(0);
+  __syncthreads();
 
-  for (int e = tid; e < tile_elems; e += 64 * NW) tileb[e] = T(0);
-
-  const int slo = wave * a.slab;
-  const int shi = min(slo + a.slab, tl[D - 1]);
+  // kernel-value lane of this thread: dim kd, index kj
+  const int kd = lane >> SH;
+  const int kj = lane & (WMAX - 1);
 
   const int nc0 = a.maxc[0];
   const int nc1 = D >= 2 ? a.maxc[1] : 1;
   const int nc2 = D >= 3 ? a.maxc[2] : 1;
   for (int c2 = 0; c2 < nc2; ++c2) {
-    int bz = D >= 3 ? cand[2 * a.cstride + t[2] * a.maxc[2] + c2] : 0;
+    const int bz = D >= 3 ? cand[2 * a.cstride + t[2] * a.maxc[2] + c2] : 0;
     if (bz < 0) continue;
     for (int c1 = 0; c1 < nc1; ++c1) {
-      int by = D >= 2 ? cand[1 * a.cstride + t[1] * a.maxc[1] + c1] : 0;
+      const int by = D >= 2 ? cand[1 * a.cstride + t[1] * a.maxc[1] + c1] : 0;
       if (by < 0) continue;
       for (int c0 = 0; c0 < nc0; ++c0) {
-        int bx = cand[t[0] * a.maxc[0] + c0];
+        const int bx = cand[t[0] * a.maxc[0] + c0];
         if (bx < 0) continue;
-        int bin = bx + a.ntile[0] * (by + a.ntile[1] * bz);
-        int beg = bin_start[bin];
-        int end = bin_start[bin + 1];
+        const int bin = bx + a.ntile[0] * (by + a.ntile[1] * bz);
+        const int beg = bin_start[bin];
+        const int end = bin_start[bin + 1];
         for (int base = beg; base < end; base += 64) {
-          __syncthreads();  // previous batch fully consumed
-          if (wave == 0) {
-            int j = base + lane;
-            bool touch = false;
-            uint32_t sc = 0;
-            T cw = T(0);
-            typename Vec4<T>::type r;
-            if (j < end) {
-              sc = scell[j];
-              r = rec[j];
-              cw = r.w;
-              touch = true;
+          const int j = base + lane;
+          uint32_t sc = 0;
+          typename Vec4<T>::type r;
+          r.x = r.y = r.z = r.w = T(0);
+          bool touch = false;
+          if (j < end) {
+            sc = scell[j];
+            r = rec[j];
+            touch = true;
 #pragma unroll
-              for (int q = 0; q < D; ++q) {
-                int s = (int)((sc >> (q * a.cbits)) & cmask);
-                int u = s - t0[q];
-                if (u < 0) u += a.nf[q];
-                touch = touch && (u < tl[q] || u + w > a.nf[q]);
-              }
+            for (int q = 0; q < D; ++q) {
+              int s = (int)((sc >> (q * a.cbits)) & cmask);
+              int u = s - t0[q];
+              if (u < 0) u += a.nf[q];
+              touch = touch && (u < tl[q] || u + w > a.nf[q]);
             }
-            unsigned long long m = __ballot(touch);
-            int pos = __popcll(m & ((1ull << lane) - 1ull));
-            if (touch) {
-              ps[pos] = sc;
-              pcw[pos] = cw;
-#pragma unroll
-              for (int q = 0; q < D; ++q) {
-                T off = rec_comp<T, D>(r, q);
-                T* dst = kv + (pos * D + q) * WMAX;
-#pragma unroll
-                for (int jj = 0; jj < WMAX; ++jj)
-                  dst[jj] = jj < w ? es_phi<T>((T(jj) - off) * a.k.two_over_w, a.k) : T(0);
-              }
-            }
-            if (lane == 0) s_np = __popcll(m);
           }
-          __syncthreads();
-          const int np = s_np;
-          for (int p = 0; p < np; ++p) {
-            const uint32_t sc = ps[p];
-            const T cw = pcw[p];
-            const T* kvp = kv + p * D * WMAX;
+          unsigned long long m = __ballot(touch);
+          while (m) {
+            const int p = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t scp = lane_read_u(sc, p);
+            const T cw = lane_read(r.w, p);
+            const T offx = lane_read(r.x, p);
+            const T offy = D >= 2 ? lane_read(r.y, p) : T(0);
+            const T offz = D >= 3 ? lane_read(r.z, p) : T(0);
+            // kernel values: lane (kd, kj) holds phi((kj - off_kd) * 2/w)
+            const T offk = kd == 0 ? offx : (kd == 1 ? offy : offz);
+            const T kv = (kd < D && kj < w) ? es_phi<T>((T(kj) - offk) * a.k.two_over_w, a.k) : T(0);
             int s[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) s[q] = q < D ? (int)((sc >> (q * a.cbits)) & cmask) : 0;
+            for (int q = 0; q < 3; ++q) s[q] = q < D ? (int)((scp >> (q * a.cbits)) & cmask) : 0;
+            int uz0 = 0;
             if (D == 3) {
-              // wave-uniform z range inside this wave's slab
-              int uz0 = s[2] - t0[2];
+              uz0 = s[2] - t0[2];
               if (uz0 < 0) uz0 += a.nf[2];
-              bool any = false;
-              for (int jc = 0; jc < w; ++jc) {
-                int uz = uz0 + jc;
-                if (uz >= a.nf[2]) uz -= a.nf[2];
-                any = any || (uz >= slo && uz < shi);
-              }
-              if (!any) continue;
-              constexpr int SH = Lanes<WMAX>::SH;
-              constexpr int NCH = (WMAX * WMAX + 63) / 64;
+            }
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) {
-                int pr = lane + 64 * ch;
-                int ja = pr & (WMAX - 1);
-                int jb = pr >> SH;
-                bool act = ja < w && jb < w;
-                int ux = s[0] + ja - t0[0];
-                if (ux >= a.nf[0]) ux -= a.nf[0];
-                if (ux < 0) ux += a.nf[0];
-                int uy = s[1] + jb - t0[1];
+            for (int ch = 0; ch < NCH; ++ch) {
+              const int pr = lane + 64 * ch;
+              const int ja = D == 1 ? pr : (pr & (WMAX - 1));
+              const int jb = D == 1 ? 0 : (pr >> SH);
+              bool act = ja < w && (D == 1 || jb < w);
+              int ux = s[0] + ja - t0[0];
+              if (ux >= a.nf[0]) ux -= a.nf[0];
+              if (ux < 0) ux += a.nf[0];
+              act = act && ux < tl[0];
+              int uy = 0;
+              if (D >= 2) {
+                uy = s[1] + jb - t0[1];
                 if (uy >= a.nf[1]) uy -= a.nf[1];
                 if (uy < 0) uy += a.nf[1];
-                act = act && ux < tl[0] && uy < tl[1];
-                T v = kvp[ja] * kvp[WMAX + (jb < WMAX ? jb : 0)] * cw;
-                T* rowp = tileb + uy * a.sx + ux;
+                act = act && uy < tl[1];
+              }
+              const T wx = lane_get(kv, ja < WMAX ? ja : 0);
+              T v = wx * cw;
+              if (D >= 2) v *= lane_get(kv, WMAX + (jb < WMAX ? jb : 0));
+              T* rowp = tileb + uy * a.sx + ux;
+              if (D == 3) {
 #pragma unroll
                 for (int jc = 0; jc < WMAX; ++jc) {
                   if (jc < w) {
                     int uz = uz0 + jc;
                     if (uz >= a.nf[2]) uz -= a.nf[2];
-                    if (uz >= slo && uz < shi && act) rowp[uz * ty * a.sx] += v * kvp[2 * WMAX + jc];
+                    const T wz = lane_read(kv, 2 * WMAX + jc);
+                    if (uz < tl[2] && act) rowp[uz * zstride] += v * wz;
                   }
                 }
+              } else {
+                if (act) rowp[0] += v;
               }
-            } else if (D == 2) {
-              constexpr int SH = Lanes<WMAX>::SH;
-              constexpr int NCH = (WMAX * WMAX + 63) / 64;
-#pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) {
-                int pr = lane + 64 * ch;
-                int ja = pr & (WMAX - 1);
-                int jb = pr >> SH;
-                bool act = ja < w && jb < w;
-                int ux = s[0] + ja - t0[0];
-                if (ux >= a.nf[0]) ux -= a.nf[0];
-                if (ux < 0) ux += a.nf[0];
-                int uy = s[1] + jb - t0[1];
-                if (uy >= a.nf[1]) uy -= a.nf[1];
-                if (uy < 0) uy += a.nf[1];
-                act = act && ux < tl[0] && uy >= slo && uy < shi;
-                if (act) tileb[uy * a.sx + ux] += kvp[ja] * kvp[WMAX + jb] * cw;
-              }
-            } else {
-              int ja = lane;
-              bool act = ja < w;
-              int ux = s[0] + ja - t0[0];
-              if (ux >= a.nf[0]) ux -= a.nf[0];
-              if (ux < 0) ux += a.nf[0];
-              act = act && ux >= slo && ux < shi;
-              if (act) tileb[ux] += kvp[ja < WMAX ? ja : 0] * cw;
             }
           }
         }
@@ -501,20 +542,24 @@ __global__ void __launch_bounds__(64 * NW)
   const int ny = D >= 2 ? tl[1] : 1;
   const int nz = D >= 3 ? tl[2] : 1;
   const int tot = nx * ny * nz;
-  for (int e = tid; e < tot; e += 64 * NW) {
+  for (int e = lane; e < tot; e += 64) {
     int ux = e % nx;
-    int r = e / nx;
-    int uy = r % ny;
-    int uz = r / ny;
+    int rr = e / nx;
+    int uy = rr % ny;
+    int uz = rr / ny;
     int64_t gidx = ((int64_t)(t0[2] + uz) * a.nf[1] + (t0[1] + uy)) * a.nf[0] + (t0[0] + ux);
     grid[gidx] = tileb[(uz * ty + uy) * a.sx + ux];
   }
 }
 
+// (register-tile 3D spreader is defined after the interpolation structs)
+
 template <typename T>
 void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
   geom = g;
   es = e;
+  if (e.w > 16) throw InvalidArgument("kernel width > 16");
+  if (g.d >= 1 && e.w > 8 && g.d == 1 && e.w > 64) throw InvalidArgument("kernel width too large");
   // candidate bins per tile and dimension: bins holding a first cell s with
   // [s, s+w) (periodic) intersecting the tile.
   std::vector<std::vector<std::vector<int>>> lists(3);
@@ -545,12 +590,7 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
       for (size_t i = 0; i < lists[a][t].size(); ++i) host[a * cand_stride + t * maxc[a] + i] = lists[a][t][i];
   cand.allocate(host.size());
   cand.upload(host.data(), host.size());
-  int sx = g.tile[0] + (g.d >= 2 ? 8 : 0);
-  int ny = g.d >= 2 ? g.tile[1] : 1;
-  int nz = g.d >= 3 ? g.tile[2] : 1;
-  int wmax = e.w <= 8 ? 8 : 16;
-  lds_bytes = (int)(sizeof(T) * ((size_t)sx * ny * nz + 64 * g.d * wmax + 64) + 64 * sizeof(uint32_t));
-  nwaves = g.d == 1 ? 1 : 4;
+  lds_bytes = (int)spread_lds_bytes(g, (int)sizeof(T), e.w);
   if (lds_bytes > 160 * 1024) throw InvalidArgument("spreader tile exceeds LDS");
 }
 
@@ -566,165 +606,511 @@ static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* gr
   }
   a.cbits = g.cell_bits;
   a.w = sp.es.w;
-  a.sx = g.tile[0] + (D >= 2 ? 8 : 0);
+  a.sx = D >= 2 ? lds_row_stride(g.tile[0], (int)sizeof(T), sp.es.w) : g.tile[0];
   a.cstride = sp.cand_stride;
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
-  if (D == 1) {
-    a.slab = g.tile[0];
-    hipLaunchKernelGGL((k_spread<T, D, WMAX, 1>), dim3(g.nbins), dim3(64), sp.lds_bytes, s, p.scell.get(),
-                       p.rec.get(), p.bin_start.get(), sp.cand.get(), a, grid);
-  } else {
-    a.slab = (g.tile[D - 1] + 3) / 4;
-    hipLaunchKernelGGL((k_spread<T, D, WMAX, 4>), dim3(g.nbins), dim3(256), sp.lds_bytes, s, p.scell.get(),
-                       p.rec.get(), p.bin_start.get(), sp.cand.get(), a, grid);
-  }
+  hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins), dim3(64), sp.lds_bytes, s, p.scell.get(), p.rec.get(),
+                     p.bin_start.get(), sp.cand.get(), a, grid);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
+
+template <typename T>
+static bool launch_spread3_reg(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s);
 
 template <typename T>
 void Spreader<T>::spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
   const int d = geom.d;
   const bool small = es.w <= 8;
   if (d == 3) {
+    if (launch_spread3_reg<T>(*this, p, grid, s)) return;
     small ? launch_spread<T, 3, 8>(*this, p, grid, s) : launch_spread<T, 3, 16>(*this, p, grid, s);
   } else if (d == 2) {
     small ? launch_spread<T, 2, 8>(*this, p, grid, s) : launch_spread<T, 2, 16>(*this, p, grid, s);
   } else {
-    small ? launch_spread<T, 1, 8>(*this, p, grid, s) : launch_spread<T, 1, 16>(*this, p, grid, s);
+    launch_spread<T, 1, 16>(*this, p, grid, s);
   }
 }
 
 // ---------------------------------------------------------------------------
-// interpolation: one wavefront per point
+// interpolation: the grid region of one bin staged in LDS, one wavefront per
+// point, DPP wavefront reduction
 // ---------------------------------------------------------------------------
 
 template <typename T>
 struct InterpDev {
-  int64_t n;
-  int nf[3];
+  int nf[3], tile[3], ntile[3];
   int cbits;
   int w;
+  int rx, ry, rz;  // staged region extent (tile + w - 1)
+  int srx;         // LDS row stride of the region
   int ostride, ocomp;
   double dscale[3];
   EsDev<T> k;
 };
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-
 template <typename T, int D, int WMAX, bool DERIV>
 __global__ void __launch_bounds__(256)
     k_interp(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
-             const uint32_t* __restrict__ idx, const T* __restrict__ grid, InterpDev<T> a,
-             double* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (k >= a.n) return;  // wave-uniform
+             const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
+             InterpDev<T> a, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* reg = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int w = a.w;
   const uint32_t cmask = (1u << a.cbits) - 1u;
-  const uint32_t sc = scell[k];
-  const typename Vec4<T>::type r = rec[k];
-  int s[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) s[q] = q < D ? (int)((sc >> (q * a.cbits)) & cmask) : 0;
-  T off[3] = {r.x, r.y, r.z};
-
-  // kernel values along the slowest dimension: lane j < w holds entry j
-  T zv = T(0), zd = T(0);
-  if (D == 3 && lane < w) es_phi_dphi<T>((T(lane) - off[2]) * a.k.two_over_w, a.k, zv, zd);
-
-  T c0 = T(0), c1 = T(0), c2 = T(0);
   constexpr int SH = Lanes<WMAX>::SH;
   constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
+
+  const int b = blockIdx.x;
+  const int beg = bin_start[b];
+  const int end = bin_start[b + 1];
+  if (beg == end) return;  // uniform across the workgroup
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  const int ry = D >= 2 ? a.ry : 1;
+  const int rz = D >= 3 ? a.rz : 1;
+  const int tot = a.rx * ry * rz;
+  for (int e = tid; e < tot; e += 256) {
+    int lx = e % a.rx;
+    int rr = e / a.rx;
+    int ly = rr % ry;
+    int lz = rr / ry;
+    const int gx = (t0[0] + lx) % a.nf[0];  // the region may wrap (and repeat
+    const int gy = (t0[1] + ly) % a.nf[1];  // cells on grids smaller than
+    const int gz = (t0[2] + lz) % a.nf[2];  // tile + w - 1)
+    reg[(lz * ry + ly) * a.srx + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+  }
+  __syncthreads();
+
+  const int kd = lane >> SH;
+  const int kj = lane & (WMAX - 1);
+  const int zs = ry * a.srx;
+  for (int k = beg + wave; k < end; k += 4) {
+    const uint32_t sc = scell[k];
+    const typename Vec4<T>::type r = rec[k];
+    int l0[3];
 #pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) {
-    int pr = lane + 64 * ch;
-    int ja = D == 1 ? pr : (pr & (WMAX - 1));
-    int jb = D == 1 ? 0 : (pr >> SH);
-    bool act = ja < w && (D == 1 || jb < w);
-    T px, dx, py = T(1), dy = T(0);
-    es_phi_dphi<T>((T(ja) - off[0]) * a.k.two_over_w, a.k, px, dx);
-    if (D >= 2) es_phi_dphi<T>((T(jb) - off[1]) * a.k.two_over_w, a.k, py, dy);
-    int gx = s[0] + ja;
-    if (gx >= a.nf[0]) gx -= a.nf[0];
-    int gy = D >= 2 ? s[1] + jb : 0;
-    if (D >= 2 && gy >= a.nf[1]) gy -= a.nf[1];
-    T s0 = T(0), s1 = T(0);
-    if (D == 3) {
+    for (int q = 0; q < 3; ++q) {
+      int s = q < D ? (int)((sc >> (q * a.cbits)) & cmask) : 0;
+      int u = s - t0[q];
+      if (u < 0) u += a.nf[q];  // first cell lies in this bin's tile
+      l0[q] = q < D ? u : 0;
+    }
+    const T offk = kd == 0 ? r.x : (kd == 1 ? r.y : r.z);
+    T kv = T(0), kdv = T(0);
+    if (kd < D && kj < w) es_phi_dphi<T>((T(kj) - offk) * a.k.two_over_w, a.k, kv, kdv);
+    T c0 = T(0), c1 = T(0), c2 = T(0);
 #pragma unroll
-      for (int jc = 0; jc < WMAX; ++jc) {
-        if (jc < w) {
-          T wv = __shfl(zv, jc, 64);
-          T wd = __shfl(zd, jc, 64);
-          int gz = s[2] + jc;
-          if (gz >= a.nf[2]) gz -= a.nf[2];
-          if (act) {
-            T v = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
-            s0 += v * wv;
-            s1 += v * wd;
+    for (int ch = 0; ch < NCH; ++ch) {
+      const int pr = lane + 64 * ch;
+      const int ja = D == 1 ? pr : (pr & (WMAX - 1));
+      const int jb = D == 1 ? 0 : (pr >> SH);
+      const bool act = ja < w && (D == 1 || jb < w);
+      const int jas = act ? ja : 0;  // inactive lanes read a valid address
+      const int jbs = act ? jb : 0;
+      const T px = lane_get(kv, jas);
+      const T dx = DERIV ? lane_get(kdv, jas) : T(0);
+      const T py = D >= 2 ? lane_get(kv, WMAX + jbs) : T(1);
+      const T dy = (DERIV && D >= 2) ? lane_get(kdv, WMAX + jbs) : T(0);
+      const T* rowp = reg + (l0[1] + jbs) * a.srx + (l0[0] + jas);
+      T s0 = T(0), s1 = T(0);
+      if (D == 3) {
+        const T* colp = rowp + l0[2] * zs;
+#pragma unroll
+        for (int jc = 0; jc < WMAX; ++jc) {
+          if (jc < w) {
+            const T wz = lane_read(kv, 2 * WMAX + jc);
+            const T v = act ? colp[jc * zs] : T(0);
+            s0 += v * wz;
+            if (DERIV) s1 += v * lane_read(kdv, 2 * WMAX + jc);
+          }
+        }
+      } else {
+        s0 = act ? rowp[0] : T(0);
+      }
+      if (DERIV) {
+        c0 += dx * py * s0;
+        if (D >= 2) c1 += px * dy * s0;
+        if (D >= 3) c2 += px * py * s1;
+      } else {
+        c0 += px * py * s0;
+      }
+    }
+    c0 = wave_sum63(c0);
+    if (DERIV && D >= 2) c1 = wave_sum63(c1);
+    if (DERIV && D >= 3) c2 = wave_sum63(c2);
+    if (lane == 63) {
+      const int64_t j = idx[k];
+      const double cw = (double)r.w;
+      if (DERIV) {
+        out[j * a.ostride + 0] = cw * a.dscale[0] * (double)c0;
+        if (D >= 2) out[j * a.ostride + 1] = cw * a.dscale[1] * (double)c1;
+        if (D >= 3) out[j * a.ostride + 2] = cw * a.dscale[2] * (double)c2;
+      } else {
+        out[j * a.ostride + a.ocomp] = cw * (double)c0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3D spreading, output-stationary: one wavefront owns a 16 x 16 x 8 tile held
+// in registers.  Lane (x_l = lane % 16, zq = lane / 16) owns the cells
+// x = x_l, z = 2 zq + {0, 1}, y = 0..15 (32 accumulators).  For every point
+// whose support touches the tile, lanes 0..3*WMAX-1 evaluate the 3w kernel
+// values once; each lane takes its x and z weights by ds_bpermute and the 16
+// y weights (shifted to the tile) from a small per-wavefront LDS table, and
+// accumulates 32 FMAs in registers.  No LDS read-modify-write, no atomics;
+// every cell sums its points in one fixed order (bitwise reproducible), and
+// is written to HBM once.  Requires nf[1] >= 16 + w - 1 (so the tile sees
+// one contiguous piece of any support along y).
+// ---------------------------------------------------------------------------
+
+constexpr int RT_X = 16, RT_Y = 16, RT_Z = 8;
+
+template <typename T, int WMAX>
+__global__ void __launch_bounds__(64)
+    k_spread3_reg(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                  const int* __restrict__ bin_start, const int* __restrict__ cand, SpreadDev<T> a,
+                  T* __restrict__ grid) {
+  constexpr int TAB = 2 * RT_Y + WMAX;
+  __shared__ T tab[64];
+  const int lane = threadIdx.x;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  constexpr int SH = Lanes<WMAX>::SH;
+  const int x_l = lane & 15;
+  const int zq = lane >> 4;
+
+  const int b = blockIdx.x;
+  int t[3];
+  t[0] = b % a.ntile[0];
+  t[1] = (b / a.ntile[0]) % a.ntile[1];
+  t[2] = b / (a.ntile[0] * a.ntile[1]);
+  int t0[3], tl[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    t0[q] = t[q] * a.tile[q];
+    tl[q] = min(a.tile[q], a.nf[q] - t0[q]);
+  }
+  T acc0[RT_Y], acc1[RT_Y];
+#pragma unroll
+  for (int r = 0; r < RT_Y; ++r) {
+    acc0[r] = T(0);
+    acc1[r] = T(0);
+  }
+  const int kd = lane >> SH;
+  const int kj = lane & (WMAX - 1);
+  const int z0 = 2 * zq, z1 = 2 * zq + 1;
+  // table lane i holds the weight of y-kernel index i - RT_Y
+  const int tj = lane - RT_Y;
+  const bool tvalid = tj >= 0 && tj < w && lane < TAB;
+
+  for (int c2 = 0; c2 < a.maxc[2]; ++c2) {
+    const int bz = cand[2 * a.cstride + t[2] * a.maxc[2] + c2];
+    if (bz < 0) continue;
+    for (int c1 = 0; c1 < a.maxc[1]; ++c1) {
+      const int by = cand[1 * a.cstride + t[1] * a.maxc[1] + c1];
+      if (by < 0) continue;
+      for (int c0 = 0; c0 < a.maxc[0]; ++c0) {
+        const int bx = cand[t[0] * a.maxc[0] + c0];
+        if (bx < 0) continue;
+        const int bin = bx + a.ntile[0] * (by + a.ntile[1] * bz);
+        const int beg = bin_start[bin];
+        const int end = bin_start[bin + 1];
+        for (int base = beg; base < end; base += 64) {
+          const int j = base + lane;
+          uint32_t sc = 0;
+          typename Vec4<T>::type rr;
+          rr.x = rr.y = rr.z = rr.w = T(0);
+          bool touch = false;
+          if (j < end) {
+            sc = scell[j];
+            rr = rec[j];
+            touch = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              int s = (int)((sc >> (q * a.cbits)) & cmask);
+              int u = s - t0[q];
+              if (u < 0) u += a.nf[q];
+              touch = touch && (u < tl[q] || u + w > a.nf[q]);
+            }
+          }
+          unsigned long long m = __ballot(touch);
+          while (m) {
+            const int p = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t scp = lane_read_u(sc, p);
+            const T cw = lane_read(rr.w, p);
+            const T ox = lane_read(rr.x, p), oy = lane_read(rr.y, p), oz = lane_read(rr.z, p);
+            const T offk = kd == 0 ? ox : (kd == 1 ? oy : oz);
+            const T kv = (kd < 3 && kj < w) ? es_phi<T>((T(kj) - offk) * a.k.two_over_w, a.k) : T(0);
+            const int s0 = (int)(scp & cmask);
+            const int s1 = (int)((scp >> a.cbits) & cmask);
+            const int s2 = (int)((scp >> (2 * a.cbits)) & cmask);
+            // x weight of this lane's column
+            int jx = t0[0] + x_l - s0;
+            if (jx < 0) jx += a.nf[0];
+            if (jx >= a.nf[0]) jx -= a.nf[0];
+            const T wxv = lane_get(kv, jx < WMAX ? jx : 0);
+            const T wx = (jx < w && x_l < tl[0]) ? wxv * cw : T(0);
+            // z weights of this lane's two cells
+            int jz0 = t0[2] + z0 - s2;
+            if (jz0 < 0) jz0 += a.nf[2];
+            if (jz0 >= a.nf[2]) jz0 -= a.nf[2];
+            int jz1 = jz0 + 1;
+            if (jz1 >= a.nf[2]) jz1 -= a.nf[2];
+            const T wz0v = lane_get(kv, 2 * WMAX + (jz0 < WMAX ? jz0 : 0));
+            const T wz1v = lane_get(kv, 2 * WMAX + (jz1 < WMAX ? jz1 : 0));
+            const T a0 = (jz0 < w && z0 < tl[2]) ? wx * wz0v : T(0);
+            const T a1 = (jz1 < w && z1 < tl[2]) ? wx * wz1v : T(0);
+            // y weights shifted to the tile: W(r) = tab[RT_Y - ey + r]
+            int dy = s1 - t0[1];
+            if (dy < 0) dy += a.nf[1];
+            const int ey = dy > a.nf[1] - w ? dy - a.nf[1] : dy;
+            const T tv = lane_get(kv, WMAX + (tvalid ? tj : 0));
+            tab[lane] = tvalid ? tv : T(0);
+            __builtin_amdgcn_wave_barrier();  // table written (in-order LDS) before it is read
+            const T* wrow = tab + (RT_Y - ey);
+#pragma unroll
+            for (int r = 0; r < RT_Y; ++r) {
+              const T wy = wrow[r];
+              acc0[r] += a0 * wy;
+              acc1[r] += a1 * wy;
+            }
+            __builtin_amdgcn_wave_barrier();  // reads done before the next point rewrites the table
           }
         }
       }
-    } else {
-      if (act) s0 = grid[(int64_t)gy * a.nf[0] + gx];
-    }
-    if (!act) {
-      s0 = T(0);
-      s1 = T(0);
-    }
-    if (DERIV) {
-      c0 += dx * py * s0;
-      if (D >= 2) c1 += px * dy * s0;
-      if (D >= 3) c2 += px * py * s1;
-    } else {
-      c0 += px * py * s0;
     }
   }
-  c0 = wave_sum(c0);
-  if (DERIV && D >= 2) c1 = wave_sum(c1);
-  if (DERIV && D >= 3) c2 = wave_sum(c2);
-  if (lane == 0) {
+  if (x_l < tl[0]) {
+    const int64_t gx = t0[0] + x_l;
+#pragma unroll
+    for (int r = 0; r < RT_Y; ++r) {
+      if (r < tl[1]) {
+        const int64_t row = (int64_t)(t0[1] + r) * a.nf[0] + gx;
+        if (z0 < tl[2]) grid[(int64_t)(t0[2] + z0) * a.nf[1] * a.nf[0] + row] = acc0[r];
+        if (z1 < tl[2]) grid[(int64_t)(t0[2] + z1) * a.nf[1] * a.nf[0] + row] = acc1[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3D interpolation, lane per point: the bin's grid region is staged in LDS by
+// one wavefront; each lane evaluates its point's 3w kernel values (and
+// derivatives) and sums its w^3 cells from LDS.  No cross-lane traffic.
+// ---------------------------------------------------------------------------
+
+constexpr int LANE_INTERP_THREADS = 128;
+
+template <typename T, int WMAX, bool DERIV>
+__global__ void __launch_bounds__(LANE_INTERP_THREADS)
+    k_interp3_lane(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                   const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
+                   InterpDev<T> a, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* reg = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int b = blockIdx.x;
+  const int beg = bin_start[b];
+  const int end = bin_start[b + 1];
+  if (beg == end) return;
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  const int tot = a.rx * a.ry * a.rz;
+  for (int e = tid; e < tot; e += LANE_INTERP_THREADS) {
+    int lx = e % a.rx;
+    int rr = e / a.rx;
+    int ly = rr % a.ry;
+    int lz = rr / a.ry;
+    const int gx = (t0[0] + lx) % a.nf[0];
+    const int gy = (t0[1] + ly) % a.nf[1];
+    const int gz = (t0[2] + lz) % a.nf[2];
+    reg[(lz * a.ry + ly) * a.srx + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+  }
+  __syncthreads();
+  const int zs = a.ry * a.srx;
+  for (int k = beg + tid; k < end; k += LANE_INTERP_THREADS) {
+    const uint32_t sc = scell[k];
+    const typename Vec4<T>::type r = rec[k];
+    int l0[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+      if (u < 0) u += a.nf[q];
+      l0[q] = u;
+    }
+    T wx[WMAX], dwx[WMAX], wy[WMAX], dwy[WMAX];
+#pragma unroll
+    for (int jj = 0; jj < WMAX; ++jj) {
+      wx[jj] = dwx[jj] = wy[jj] = dwy[jj] = T(0);
+      if (jj < w) {
+        if (DERIV) {
+          es_phi_dphi<T>((T(jj) - r.x) * a.k.two_over_w, a.k, wx[jj], dwx[jj]);
+          es_phi_dphi<T>((T(jj) - r.y) * a.k.two_over_w, a.k, wy[jj], dwy[jj]);
+        } else {
+          wx[jj] = es_phi<T>((T(jj) - r.x) * a.k.two_over_w, a.k);
+          wy[jj] = es_phi<T>((T(jj) - r.y) * a.k.two_over_w, a.k);
+        }
+      }
+    }
+    T gx = T(0), gy = T(0), gz = T(0);
+    const T* col = reg + (l0[2] * a.ry + l0[1]) * a.srx + l0[0];
+#pragma unroll 1
+    for (int jc = 0; jc < w; ++jc) {
+      T wz, dwz = T(0);
+      if (DERIV)
+        es_phi_dphi<T>((T(jc) - r.z) * a.k.two_over_w, a.k, wz, dwz);
+      else
+        wz = es_phi<T>((T(jc) - r.z) * a.k.two_over_w, a.k);
+      T sx0 = T(0), sx1 = T(0), sy = T(0);
+#pragma unroll
+      for (int jb = 0; jb < WMAX; ++jb) {
+        if (jb < w) {
+          const T* row = col + jb * a.srx;
+          T t0v = T(0), t1v = T(0);
+#pragma unroll
+          for (int ja = 0; ja < WMAX; ++ja) {
+            if (ja < w) {
+              const T g = row[ja];
+              t0v += g * wx[ja];
+              if (DERIV) t1v += g * dwx[ja];
+            }
+          }
+          sx0 += t0v * wy[jb];
+          if (DERIV) {
+            sx1 += t1v * wy[jb];
+            sy += t0v * dwy[jb];
+          }
+        }
+      }
+      if (DERIV) {
+        gx += sx1 * wz;
+        gy += sy * wz;
+        gz += sx0 * dwz;
+      } else {
+        gx += sx0 * wz;
+      }
+      col += zs;
+    }
     const int64_t j = idx[k];
     const double cw = (double)r.w;
     if (DERIV) {
-      out[j * a.ostride + 0] = cw * a.dscale[0] * (double)c0;
-      if (D >= 2) out[j * a.ostride + 1] = cw * a.dscale[1] * (double)c1;
-      if (D >= 3) out[j * a.ostride + 2] = cw * a.dscale[2] * (double)c2;
+      out[j * a.ostride + 0] = cw * a.dscale[0] * (double)gx;
+      out[j * a.ostride + 1] = cw * a.dscale[1] * (double)gy;
+      out[j * a.ostride + 2] = cw * a.dscale[2] * (double)gz;
     } else {
-      out[j * a.ostride + a.ocomp] = cw * (double)c0;
+      out[j * a.ostride + a.ocomp] = cw * (double)gx;
     }
   }
+}
+
+template <typename T, int WMAX>
+static void launch_spread3_reg_w(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
+  SpreadDev<T> a;
+  const GridGeom& g = sp.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.maxc[q] = sp.maxc[q];
+  }
+  a.cbits = g.cell_bits;
+  a.w = sp.es.w;
+  a.sx = 0;
+  a.cstride = sp.cand_stride;
+  a.k.beta = (T)sp.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
+  a.k.two_over_w = (T)(2.0 / sp.es.w);
+  hipLaunchKernelGGL((k_spread3_reg<T, WMAX>), dim3(g.nbins), dim3(64), 0, s, p.scell.get(), p.rec.get(),
+                     p.bin_start.get(), sp.cand.get(), a, grid);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static bool launch_spread3_reg(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
+  const GridGeom& g = sp.geom;
+  if (g.tile[0] != RT_X || g.tile[1] != RT_Y || g.tile[2] != RT_Z) return false;
+  if (g.nf[1] < RT_Y + sp.es.w - 1) return false;
+  if (sp.es.w <= 8)
+    launch_spread3_reg_w<T, 8>(sp, p, grid, s);
+  else
+    launch_spread3_reg_w<T, 16>(sp, p, grid, s);
+  return true;
+}
+
+template <typename T, int WMAX, bool DERIV>
+static void launch_interp3_lane(const SortedPoints<T>& p, const T* grid, const double* dscale, double* out,
+                                int ostride, int ocomp, hipStream_t s) {
+  InterpDev<T> a;
+  const GridGeom& g = p.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.dscale[q] = dscale ? dscale[q] : 0.0;
+  }
+  a.cbits = g.cell_bits;
+  a.w = p.es.w;
+  a.rx = g.tile[0] + a.w - 1;
+  a.ry = g.tile[1] + a.w - 1;
+  a.rz = g.tile[2] + a.w - 1;
+  a.srx = a.rx;
+  a.ostride = ostride;
+  a.ocomp = ocomp;
+  a.k.beta = (T)p.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
+  a.k.two_over_w = (T)(2.0 / p.es.w);
+  size_t lds = sizeof(T) * (size_t)a.rx * a.ry * a.rz;
+  if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS");
+  hipLaunchKernelGGL((k_interp3_lane<T, WMAX, DERIV>), dim3(g.nbins), dim3(LANE_INTERP_THREADS), lds, s,
+                     p.scell.get(), p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+  FRESCO_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T, int D, int WMAX>
 static void launch_interp(const SortedPoints<T>& p, const T* grid, bool deriv, const double* dscale, double* out,
                           int ostride, int ocomp, hipStream_t s) {
   InterpDev<T> a;
-  a.n = p.n;
+  const GridGeom& g = p.geom;
   for (int q = 0; q < 3; ++q) {
-    a.nf[q] = p.geom.nf[q];
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
     a.dscale[q] = dscale ? dscale[q] : 0.0;
   }
-  a.cbits = p.geom.cell_bits;
+  a.cbits = g.cell_bits;
   a.w = p.es.w;
+  a.rx = g.tile[0] + a.w - 1;
+  a.ry = D >= 2 ? g.tile[1] + a.w - 1 : 1;
+  a.rz = D >= 3 ? g.tile[2] + a.w - 1 : 1;
+  a.srx = D >= 2 ? lds_row_stride(a.rx, (int)sizeof(T), a.w) : a.rx;
   a.ostride = ostride;
   a.ocomp = ocomp;
   a.k.beta = (T)p.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
   a.k.two_over_w = (T)(2.0 / p.es.w);
-  dim3 grd(blocks_for(p.n, 4)), blk(256);
+  size_t lds = interp_lds_bytes(g, (int)sizeof(T), a.w);
+  if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS; use a smaller tile");
+  dim3 grd(g.nbins), blk(256);
   if (deriv)
-    hipLaunchKernelGGL((k_interp<T, D, WMAX, true>), grd, blk, 0, s, p.scell.get(), p.rec.get(), p.idx.get(), grid,
-                       a, out);
+    hipLaunchKernelGGL((k_interp<T, D, WMAX, true>), grd, blk, lds, s, p.scell.get(), p.rec.get(), p.idx.get(),
+                       p.bin_start.get(), grid, a, out);
   else
-    hipLaunchKernelGGL((k_interp<T, D, WMAX, false>), grd, blk, 0, s, p.scell.get(), p.rec.get(), p.idx.get(),
-                       grid, a, out);
+    hipLaunchKernelGGL((k_interp<T, D, WMAX, false>), grd, blk, lds, s, p.scell.get(), p.rec.get(), p.idx.get(),
+                       p.bin_start.get(), grid, a, out);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 
@@ -734,8 +1120,18 @@ void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const doub
   const int d = p.geom.d;
   const bool small = p.es.w <= 8;
   if (d == 3) {
-    small ? launch_interp<T, 3, 8>(p, grid, deriv, dscale, out, ostride, ocomp, s)
-          : launch_interp<T, 3, 16>(p, grid, deriv, dscale, out, ostride, ocomp, s);
+    // lane per point, except the FP64 derivative with w > 8 (its 4 x 16 FP64
+    // weight arrays would spill): wavefront per point there
+    if (small) {
+      deriv ? launch_interp3_lane<T, 8, true>(p, grid, dscale, out, ostride, ocomp, s)
+            : launch_interp3_lane<T, 8, false>(p, grid, dscale, out, ostride, ocomp, s);
+    } else if (!deriv) {
+      launch_interp3_lane<T, 16, false>(p, grid, dscale, out, ostride, ocomp, s);
+    } else if (sizeof(T) == 4) {
+      launch_interp3_lane<T, 16, true>(p, grid, dscale, out, ostride, ocomp, s);
+    } else {
+      launch_interp<T, 3, 16>(p, grid, deriv, dscale, out, ostride, ocomp, s);
+    }
   } else if (d == 2) {
     small ? launch_interp<T, 2, 8>(p, grid, deriv, dscale, out, ostride, ocomp, s)
           : launch_interp<T, 2, 16>(p, grid, deriv, dscale, out, ostride, ocomp, s);

@@ -1,12 +1,13 @@
 // Non-uniform FFT building blocks on gfx950:
 //   * SortedPoints : coordinate map -> fine-grid cell + offset, bin (tile) key,
 //                    stable radix sort by bin, bin offsets, sorted point records.
-//   * Spreader     : type-1 spreading, owner-computes tiles accumulated in LDS
-//                    (each fine-grid cell is written exactly once, no atomics,
-//                    fixed summation order -> bitwise deterministic).
-//   * interpolate  : type-2 interpolation, one wavefront per point with a
-//                    wavefront reduction; either the value or the gradient
-//                    through the kernel derivative.
+//   * Spreader     : type-1 spreading, one wavefront per owner-computes tile
+//                    accumulated in LDS with in-order non-returning LDS adds
+//                    (each fine-grid cell written to HBM exactly once, no
+//                    global atomics, fixed summation order -> reproducible).
+//   * interpolate  : type-2 interpolation from the bin's grid region staged
+//                    in LDS, one wavefront per point, DPP reduction; either
+//                    the value or the gradient through the kernel derivative.
 //   * FftPlan      : rocFFT plan + work buffer + execution info.
 #pragma once
 
@@ -64,6 +65,9 @@ struct GridGeom {
 // Choose tiles for a fine grid (power-of-two tiles, enough of them to fill
 // the GPU, LDS budget respected).
 GridGeom make_geom(int d, const int* nf, int w, int elem_bytes);
+int lds_row_stride(int n, int elem_bytes, int w);
+size_t spread_lds_bytes(const GridGeom& g, int elem_bytes, int w);
+size_t interp_lds_bytes(const GridGeom& g, int elem_bytes, int w);
 
 // Sorted non-uniform point set shared by the spreader and the interpolator.
 template <typename T>
@@ -98,7 +102,6 @@ class Spreader {
   DeviceBuffer<int> cand;  // [3][maxtile][MAXC] candidate bins per tile, -1 padded
   int cand_stride = 0;     // entries per dimension block
   int lds_bytes = 0;
-  int nwaves = 4;
 };
 
 // Interpolation of a real fine grid at the sorted points.

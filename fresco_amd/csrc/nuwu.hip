@@ -174,7 +174,9 @@ class NUwU final : public Potential {
       L_[a] = L[a];
     }
     if (sigma <= 0) sigma = 2.0;
-    es_ = es_params(eps, sigma);
+    // the adjoint path interpolates the kernel derivative: one extra width
+    // (two in 1D) keeps its gradient at the accuracy the eps rule gives values
+    es_ = es_params(eps, sigma, plain_ ? 0 : (d == 1 ? 2 : 1));
     int nf[3] = {1, 1, 1};
     for (int a = 0; a < d; ++a) nf[a] = next_smooth_even((int)std::ceil(sigma * nm_), es_.w);
     geom_ = make_geom(d, nf, es_.w, (int)sizeof(T));

@@ -165,12 +165,16 @@ def test_c1_config_lbfgs_teacher_forced(mods):
     pot = NUwU(np.ones(N), kp, Sk, V, np.ones(2), 1e-9, precision="fp64")
     ref = D.NUwU(np.ones(N), kp, Sk, V, np.ones(2))
     opt = LBFGS_CPP(x0, pot, tol=1e-8 / np.sqrt(2 * N), M=10, nsteps=2000)
-    E0 = None
-    for block in range(4):
-        res = opt.run(50)
+    E0, g0 = ref.get_energy_gradient(x0)
+    # NUFFT errors are relative to the input scale, not to the vanishing
+    # energy near the stealthy optimum: compare against the initial scale
+    while True:
+        res = opt.run(20)
         E, g = ref.get_energy_gradient(res.coords)
-        E0 = E0 or E
-        print("C1 it", res.nsteps, "E", res.energy, "oracle", E, flush=True)
-        assert abs(res.energy - E) <= 1e-7 * abs(E)
-        assert np.linalg.norm(res.grad - g) <= 1e-6 * np.linalg.norm(g)
-    assert res.energy < 1e-2 * pot.get_energy(x0)
+        print("C1 it", res.nsteps, "E", res.energy, "oracle", E, "rms", res.rms, flush=True)
+        assert abs(res.energy - E) <= 1e-8 * abs(E0)
+        assert np.linalg.norm(res.grad - g) <= 1e-8 * np.linalg.norm(g0)
+        if res.success or res.nsteps >= 2000:
+            break
+    assert res.success
+    assert np.linalg.norm(g) < 1e-7
