@@ -50,10 +50,16 @@ struct EsDev {
 __device__ __forceinline__ double es_exp(double x) { return exp(x); }
 __device__ __forceinline__ float es_exp(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// float32: the hardware v_sqrt_f32 / v_rcp_f32 (about 1 ulp), far below the
+// kernel's own truncation error; float64 keeps the IEEE operations.
 template <typename T>
 __device__ __forceinline__ T es_sqrt(T a) { return sqrt(a); }
 template <>
-__device__ __forceinline__ float es_sqrt<float>(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ float es_sqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+template <typename T>
+__device__ __forceinline__ T es_rcp(T a) { return T(1) / a; }
+template <>
+__device__ __forceinline__ float es_rcp<float>(float a) { return __builtin_amdgcn_rcpf(a); }
 
 // phi(z)
 template <typename T>
@@ -71,7 +77,7 @@ __device__ __forceinline__ void es_phi_dphi(T z, const EsDev<T>& k, T& v, T& dv)
   if (a > T(0)) {
     T r = es_sqrt(a);
     v = es_exp(k.bexp * (r - T(1)));
-    dv = -k.beta * z / r * v;
+    dv = -k.beta * z * es_rcp(r) * v;
   } else {
     v = T(0);
     dv = T(0);

@@ -765,8 +765,9 @@ __global__ void __launch_bounds__(256)
 // x = x_l, z = 2 zq + {0, 1}, y = 0..15 (32 accumulators).  For every point
 // whose support touches the tile, lanes 0..3*WMAX-1 evaluate the 3w kernel
 // values once; each lane takes its x and z weights by ds_bpermute and the 16
-// y weights (shifted to the tile) from a small per-wavefront LDS table, and
-// accumulates 32 FMAs in registers.  No LDS read-modify-write, no atomics;
+// y weights (shifted to the tile) from a small per-wavefront LDS table read by
+// broadcast, and accumulates up to 32 FMAs in registers (the half of the
+// y-column the support misses is skipped).  No LDS read-modify-write, no atomics;
 // every cell sums its points in one fixed order (bitwise reproducible), and
 // is written to HBM once.  Requires nf[1] >= 16 + w - 1 (so the tile sees
 // one contiguous piece of any support along y).
@@ -851,38 +852,53 @@ __global__ void __launch_bounds__(64)
             const T ox = lane_read(rr.x, p), oy = lane_read(rr.y, p), oz = lane_read(rr.z, p);
             const T offk = kd == 0 ? ox : (kd == 1 ? oy : oz);
             const T kv = (kd < 3 && kj < w) ? es_phi<T>((T(kj) - offk) * a.k.two_over_w, a.k) : T(0);
-            const int s0 = (int)(scp & cmask);
-            const int s1 = (int)((scp >> a.cbits) & cmask);
-            const int s2 = (int)((scp >> (2 * a.cbits)) & cmask);
+            // first cell relative to the tile start, wrapped into
+            // [-(w-1), tile) (wavefront-uniform, scalar arithmetic)
+            int e3[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              int dq = (int)((scp >> (q * a.cbits)) & cmask) - t0[q];
+              if (dq < 0) dq += a.nf[q];
+              e3[q] = dq > a.nf[q] - w ? dq - a.nf[q] : dq;
+            }
             // x weight of this lane's column
-            int jx = t0[0] + x_l - s0;
-            if (jx < 0) jx += a.nf[0];
-            if (jx >= a.nf[0]) jx -= a.nf[0];
-            const T wxv = lane_get(kv, jx < WMAX ? jx : 0);
-            const T wx = (jx < w && x_l < tl[0]) ? wxv * cw : T(0);
+            const int jx = x_l - e3[0];
+            const bool vx = (unsigned)jx < (unsigned)w && x_l < tl[0];
+            const T wxv = lane_get(kv, vx ? jx : 0);
+            const T wx = vx ? wxv * cw : T(0);
             // z weights of this lane's two cells
-            int jz0 = t0[2] + z0 - s2;
-            if (jz0 < 0) jz0 += a.nf[2];
-            if (jz0 >= a.nf[2]) jz0 -= a.nf[2];
-            int jz1 = jz0 + 1;
-            if (jz1 >= a.nf[2]) jz1 -= a.nf[2];
-            const T wz0v = lane_get(kv, 2 * WMAX + (jz0 < WMAX ? jz0 : 0));
-            const T wz1v = lane_get(kv, 2 * WMAX + (jz1 < WMAX ? jz1 : 0));
-            const T a0 = (jz0 < w && z0 < tl[2]) ? wx * wz0v : T(0);
-            const T a1 = (jz1 < w && z1 < tl[2]) ? wx * wz1v : T(0);
-            // y weights shifted to the tile: W(r) = tab[RT_Y - ey + r]
-            int dy = s1 - t0[1];
-            if (dy < 0) dy += a.nf[1];
-            const int ey = dy > a.nf[1] - w ? dy - a.nf[1] : dy;
+            const int jz0 = z0 - e3[2];
+            const int jz1 = jz0 + 1;
+            const bool vz0 = (unsigned)jz0 < (unsigned)w && z0 < tl[2];
+            const bool vz1 = (unsigned)jz1 < (unsigned)w && z1 < tl[2];
+            const T wz0v = lane_get(kv, 2 * WMAX + (vz0 ? jz0 : 0));
+            const T wz1v = lane_get(kv, 2 * WMAX + (vz1 ? jz1 : 0));
+            const T a0 = vz0 ? wx * wz0v : T(0);
+            const T a1 = vz1 ? wx * wz1v : T(0);
+            // y weights shifted to the tile, W(r) = kv_y[r - ey], through a
+            // per-wavefront LDS table read by broadcast
+            const int ey = e3[1];
             const T tv = lane_get(kv, WMAX + (tvalid ? tj : 0));
             tab[lane] = tvalid ? tv : T(0);
             __builtin_amdgcn_wave_barrier();  // table written (in-order LDS) before it is read
             const T* wrow = tab + (RT_Y - ey);
+            // the support covers y = ey .. ey + w - 1: skip a half of the
+            // column that it misses (wavefront-uniform branches)
+            if (ey < RT_Y / 2) {
 #pragma unroll
-            for (int r = 0; r < RT_Y; ++r) {
-              const T wy = wrow[r];
-              acc0[r] += a0 * wy;
-              acc1[r] += a1 * wy;
+              for (int r = 0; r < RT_Y / 2; ++r) {
+                const T wy = wrow[r];
+                acc0[r] += a0 * wy;
+                acc1[r] += a1 * wy;
+              }
+            }
+            if (ey + w > RT_Y / 2) {
+#pragma unroll
+              for (int r = RT_Y / 2; r < RT_Y; ++r) {
+                const T wy = wrow[r];
+                acc0[r] += a0 * wy;
+                acc1[r] += a1 * wy;
+              }
             }
             __builtin_amdgcn_wave_barrier();  // reads done before the next point rewrites the table
           }
@@ -1080,6 +1096,210 @@ static void launch_interp3_lane(const SortedPoints<T>& p, const T* grid, const d
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// 3D interpolation, eight lanes per point (w <= 8): the bin's grid region is
+// staged in LDS by a 256-thread workgroup (tile + 7 cells per dimension);
+// lane ja of an 8-lane group sums the 8 x 8 cells of its x offset (weights w_y w_z and their derivatives, computed
+// per lane), multiplies by its own x weight, and the group reduces with three
+// DPP steps (result in the group's last lane).  Consecutive lanes read
+// consecutive LDS words.
+// ---------------------------------------------------------------------------
+
+constexpr int G8_THREADS = 256;
+
+// out[j] = value of lane (lane & ~7) + j, j < 8 (ds_swizzle bit mode:
+// and_mask 0x18, or_mask j)
+template <int J>
+__device__ __forceinline__ int swz8(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, 0x18 | (J << 5));
+}
+template <typename T>
+__device__ __forceinline__ void group8_bcast(T v, T* out);
+template <>
+__device__ __forceinline__ void group8_bcast<float>(float v, float* out) {
+  const int b = __float_as_int(v);
+  out[0] = __int_as_float(swz8<0>(b));
+  out[1] = __int_as_float(swz8<1>(b));
+  out[2] = __int_as_float(swz8<2>(b));
+  out[3] = __int_as_float(swz8<3>(b));
+  out[4] = __int_as_float(swz8<4>(b));
+  out[5] = __int_as_float(swz8<5>(b));
+  out[6] = __int_as_float(swz8<6>(b));
+  out[7] = __int_as_float(swz8<7>(b));
+}
+template <>
+__device__ __forceinline__ void group8_bcast<double>(double v, double* out) {
+  const long long bb = __double_as_longlong(v);
+  const int lo = (int)(bb & 0xffffffffLL), hi = (int)(bb >> 32);
+  int l[8], h[8];
+  l[0] = swz8<0>(lo); h[0] = swz8<0>(hi);
+  l[1] = swz8<1>(lo); h[1] = swz8<1>(hi);
+  l[2] = swz8<2>(lo); h[2] = swz8<2>(hi);
+  l[3] = swz8<3>(lo); h[3] = swz8<3>(hi);
+  l[4] = swz8<4>(lo); h[4] = swz8<4>(hi);
+  l[5] = swz8<5>(lo); h[5] = swz8<5>(hi);
+  l[6] = swz8<6>(lo); h[6] = swz8<6>(hi);
+  l[7] = swz8<7>(lo); h[7] = swz8<7>(hi);
+  for (int j = 0; j < 8; ++j) out[j] = __longlong_as_double(((long long)h[j] << 32) | (unsigned int)l[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ T sum8_last(T v) {
+  v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x114>(v);  // row_shr:4 (lanes 8g+4..8g+7 add the lower quad)
+  return v;
+}
+
+template <typename T, bool DERIV>
+__global__ void __launch_bounds__(G8_THREADS)
+    k_interp3_g8(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                 const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
+                 InterpDev<T> a, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* reg = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int b = blockIdx.x;
+  const int beg = bin_start[b];
+  const int end = bin_start[b + 1];
+  if (beg == end) return;
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  // region rows of up to 32 cells: thread (row slot, x)
+  {
+    const int lx = tid & 31;
+    int gx = t0[0] + lx;
+    if (gx >= a.nf[0]) gx -= a.nf[0];
+    if (gx >= a.nf[0]) gx -= a.nf[0];
+    const int nrows = a.ry * a.rz;
+    for (int row = tid >> 5; row < nrows; row += G8_THREADS / 32) {
+      const int ly = row % a.ry;
+      const int lz = row / a.ry;
+      int gy = t0[1] + ly;
+      if (gy >= a.nf[1]) gy -= a.nf[1];
+      if (gy >= a.nf[1]) gy -= a.nf[1];
+      int gz = t0[2] + lz;
+      if (gz >= a.nf[2]) gz -= a.nf[2];
+      if (gz >= a.nf[2]) gz -= a.nf[2];
+      if (lx < a.rx) reg[(lz * a.ry + ly) * a.srx + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+    }
+  }
+  __syncthreads();
+  const int ja = tid & 7;
+  const int grp = tid >> 3;
+  const int zs = a.ry * a.srx;
+  const int npts = end - beg;
+  // every group runs the same number of iterations (DPP needs all lanes)
+  const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
+  for (int it = 0; it < iters; ++it) {
+    const int k = beg + it * (G8_THREADS / 8) + grp;
+    const bool valid = k < end;
+    const int kk = valid ? k : beg;
+    const uint32_t sc = scell[kk];
+    const typename Vec4<T>::type r = rec[kk];
+    int l0[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+      if (u < 0) u += a.nf[q];
+      l0[q] = u;
+    }
+    // lane ja evaluates (phi, phi') of index ja along x, y and z; the group
+    // shares the y and z values by ds_swizzle (broadcast of lane j of the
+    // 8-lane group)
+    T wxa = T(0), dwxa = T(0), wya = T(0), dwya = T(0), wza = T(0), dwza = T(0);
+    if (ja < w) {
+      if (DERIV) {
+        es_phi_dphi<T>((T(ja) - r.x) * a.k.two_over_w, a.k, wxa, dwxa);
+        es_phi_dphi<T>((T(ja) - r.y) * a.k.two_over_w, a.k, wya, dwya);
+        es_phi_dphi<T>((T(ja) - r.z) * a.k.two_over_w, a.k, wza, dwza);
+      } else {
+        wxa = es_phi<T>((T(ja) - r.x) * a.k.two_over_w, a.k);
+        wya = es_phi<T>((T(ja) - r.y) * a.k.two_over_w, a.k);
+        wza = es_phi<T>((T(ja) - r.z) * a.k.two_over_w, a.k);
+      }
+    }
+    T wy[8], dwy[8], wz[8], dwz[8];
+    group8_bcast<T>(wya, wy);
+    group8_bcast<T>(wza, wz);
+    if (DERIV) {
+      group8_bcast<T>(dwya, dwy);
+      group8_bcast<T>(dwza, dwz);
+    }
+    const T* col = reg + (l0[2] * a.ry + l0[1]) * a.srx + l0[0] + ja;
+    // all 8 x 8 cells are read (weights beyond w are zero and the region
+    // is staged 8 - 1 cells wider than the tile), so the loads can be batched
+    T sA = T(0), sB = T(0), sC = T(0);
+#pragma unroll
+    for (int jc = 0; jc < 8; ++jc) {
+      T pA = T(0), pB = T(0);
+#pragma unroll
+      for (int jb = 0; jb < 8; ++jb) {
+        const T g = col[jc * zs + jb * a.srx];
+        pA += g * wy[jb];
+        if (DERIV) pB += g * dwy[jb];
+      }
+      sA += pA * wz[jc];
+      if (DERIV) {
+        sB += pB * wz[jc];
+        sC += pA * dwz[jc];
+      }
+    }
+    T vx, vy = T(0), vz = T(0);
+    if (DERIV) {
+      vx = sum8_last(dwxa * sA);
+      vy = sum8_last(wxa * sB);
+      vz = sum8_last(wxa * sC);
+    } else {
+      vx = sum8_last(wxa * sA);
+    }
+    if (ja == 7 && valid) {
+      const int64_t j = idx[k];
+      const double cw = (double)r.w;
+      if (DERIV) {
+        out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
+        out[j * a.ostride + 1] = cw * a.dscale[1] * (double)vy;
+        out[j * a.ostride + 2] = cw * a.dscale[2] * (double)vz;
+      } else {
+        out[j * a.ostride + a.ocomp] = cw * (double)vx;
+      }
+    }
+  }
+}
+
+template <typename T, bool DERIV>
+static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const double* dscale, double* out,
+                              int ostride, int ocomp, hipStream_t s) {
+  InterpDev<T> a;
+  const GridGeom& g = p.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.dscale[q] = dscale ? dscale[q] : 0.0;
+  }
+  a.cbits = g.cell_bits;
+  a.w = p.es.w;
+  a.rx = g.tile[0] + 8 - 1;
+  a.ry = g.tile[1] + 8 - 1;
+  a.rz = g.tile[2] + 8 - 1;
+  a.srx = a.rx;
+  a.ostride = ostride;
+  a.ocomp = ocomp;
+  a.k.beta = (T)p.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
+  a.k.two_over_w = (T)(2.0 / p.es.w);
+  size_t lds = sizeof(T) * (size_t)a.rx * a.ry * a.rz;
+  if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS");
+  hipLaunchKernelGGL((k_interp3_g8<T, DERIV>), dim3(g.nbins), dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
+                     p.idx.get(), p.bin_start.get(), grid, a, out);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T, int D, int WMAX>
 static void launch_interp(const SortedPoints<T>& p, const T* grid, bool deriv, const double* dscale, double* out,
                           int ostride, int ocomp, hipStream_t s) {
@@ -1122,7 +1342,11 @@ void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const doub
   if (d == 3) {
     // lane per point, except the FP64 derivative with w > 8 (its 4 x 16 FP64
     // weight arrays would spill): wavefront per point there
-    if (small) {
+    const bool fits = p.geom.tile[0] + 8 - 1 <= 32;
+    if (small && fits) {
+      deriv ? launch_interp3_g8<T, true>(p, grid, dscale, out, ostride, ocomp, s)
+            : launch_interp3_g8<T, false>(p, grid, dscale, out, ostride, ocomp, s);
+    } else if (small) {
       deriv ? launch_interp3_lane<T, 8, true>(p, grid, dscale, out, ostride, ocomp, s)
             : launch_interp3_lane<T, 8, false>(p, grid, dscale, out, ostride, ocomp, s);
     } else if (!deriv) {
