@@ -214,7 +214,8 @@ __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, 
   cell = (int)ic;
 }
 
-__global__ void k_bin_keys(const double* __restrict__ x, int64_t n, MapDev m, uint32_t* __restrict__ key,
+template <typename K>
+__global__ void k_bin_keys(const double* __restrict__ x, int64_t n, MapDev m, K* __restrict__ key,
                            uint32_t* __restrict__ idx) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -228,17 +229,18 @@ __global__ void k_bin_keys(const double* __restrict__ x, int64_t n, MapDev m, ui
     b += mul * (uint32_t)t;
     mul *= (uint32_t)m.ntile[a];
   }
-  key[j] = b;
+  key[j] = (K)b;
   idx[j] = (uint32_t)j;
 }
 
-__global__ void k_bin_start(const uint32_t* __restrict__ keys, int64_t n, int nbins, int* __restrict__ start) {
+template <typename K>
+__global__ void k_bin_start(const K* __restrict__ keys, int64_t n, int nbins, int* __restrict__ start) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nbins) return;
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     int64_t mid = (lo + hi) >> 1;
-    if (keys[mid] < (uint32_t)b)
+    if ((int)keys[mid] < b)
       lo = mid + 1;
     else
       hi = mid;
@@ -293,16 +295,25 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   geom = g;
   es = e;
   n = npts;
-  key.allocate(n);
-  key_sorted.allocate(n);
+  // 16-bit bin keys when they fit: rocPRIM then runs its onesweep radix sort
+  small_keys = g.nbins + 1 < 65535;
   idx_in.allocate(n);
   idx.allocate(n);
   bin_start.allocate(g.nbins + 1);
   scell.allocate(n);
   rec.allocate(n);
   size_t bytes = 0;
-  FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.get(), key_sorted.get(), idx_in.get(),
-                                                      idx.get(), (int)n, 0, g.key_bits));
+  if (small_keys) {
+    key16.allocate(n);
+    key16_sorted.allocate(n);
+    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key16.get(), key16_sorted.get(),
+                                                        idx_in.get(), idx.get(), (int)n, 0, g.key_bits));
+  } else {
+    key.allocate(n);
+    key_sorted.allocate(n);
+    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.get(), key_sorted.get(), idx_in.get(),
+                                                        idx.get(), (int)n, 0, g.key_bits));
+  }
   sort_tmp_bytes = std::max<size_t>(bytes, 16);
   sort_tmp.allocate(sort_tmp_bytes);
 }
@@ -311,13 +322,24 @@ template <typename T>
 void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipStream_t s) {
   MapDev m = make_map_dev(geom, es, cm);
   const unsigned thr = 256;
-  hipLaunchKernelGGL(k_bin_keys, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(), idx_in.get());
-  FRESCO_HIP_CHECK(hipGetLastError());
   size_t bytes = sort_tmp_bytes;
-  FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key.get(), key_sorted.get(),
-                                                      idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
-  hipLaunchKernelGGL(k_bin_start, dim3(blocks_for(geom.nbins + 1, thr)), dim3(thr), 0, s, key_sorted.get(), n,
-                     geom.nbins, bin_start.get());
+  if (small_keys) {
+    hipLaunchKernelGGL(k_bin_keys<uint16_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key16.get(),
+                       idx_in.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key16.get(), key16_sorted.get(),
+                                                        idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
+    hipLaunchKernelGGL(k_bin_start<uint16_t>, dim3(blocks_for(geom.nbins + 1, thr)), dim3(thr), 0, s,
+                       key16_sorted.get(), n, geom.nbins, bin_start.get());
+  } else {
+    hipLaunchKernelGGL(k_bin_keys<uint32_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(),
+                       idx_in.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key.get(), key_sorted.get(),
+                                                        idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
+    hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(geom.nbins + 1, thr)), dim3(thr), 0, s,
+                       key_sorted.get(), n, geom.nbins, bin_start.get());
+  }
   FRESCO_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, idx.get(), c, scell.get(),
                      rec.get());

@@ -82,6 +82,8 @@ class SortedPoints {
   GridGeom geom;
   EsParams es;
   DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
+  DeviceBuffer<uint16_t> key16, key16_sorted;            // used when nbins < 65535 (onesweep sort)
+  bool small_keys = false;
   DeviceBuffer<int> bin_start;                          // [nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
