@@ -101,6 +101,46 @@ int fresco_nuwu_create(const double* radii, int64_t n, int d, double K, const do
   });
 }
 
+int fresco_uwu_create(const int64_t* shape, int d, const double* Sk, const double* V, int64_t m, int precision,
+                      int exact_grad, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!shape || !Sk || !V) throw fresco_hip::InvalidArgument("null array argument");
+    if (precision < 0 || precision > 2) throw fresco_hip::InvalidArgument("unknown precision");
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_uwu(shape, d, Sk, V, m, precision, exact_grad);
+    *out = h.release();
+  });
+}
+
+int fresco_uwnu_create(const int64_t* shape, int d, const double* K, int64_t nkd, const double* Sk, const double* V,
+                       int64_t nk, const double* L, double eps, int precision, double sigma, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!shape || !K || !Sk || !V || !L) throw fresco_hip::InvalidArgument("null array argument");
+    if (precision < 0 || precision > 2) throw fresco_hip::InvalidArgument("unknown precision");
+    if (!(eps > 0)) throw fresco_hip::InvalidArgument("eps must be positive");
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_uwnu(shape, d, K, nkd, Sk, V, nk, L, eps, precision, sigma);
+    *out = h.release();
+  });
+}
+
+int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, int64_t nkd, const double* Sk,
+                        const double* V, int64_t m, const double* L, int precision, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!radii || !K || !Sk || !V || !L) throw fresco_hip::InvalidArgument("null array argument");
+    if (precision < 0 || precision > 2) throw fresco_hip::InvalidArgument("unknown precision");
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_nuwnu(radii, n, d, K, nkd, Sk, V, m, L, precision);
+    *out = h.release();
+  });
+}
+
 int fresco_combined_create(fresco_pot* const* pots, int count, fresco_pot** out) {
   return guarded([&]() {
     if (!out) throw fresco_hip::InvalidArgument("out is null");

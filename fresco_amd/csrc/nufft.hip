@@ -432,11 +432,14 @@ struct Lanes {
 // (bitwise reproducible); each fine-grid cell is written to HBM exactly once,
 // no atomics.  Used for 1D / 2D and for 3D grids too small for the
 // register-tile spreader.
-template <typename T, int D, int WMAX>
+// NC = 1: real strengths (the record's weight); NC = 2: complex strengths
+// str[2 idx[j] + {0,1}] (original point order) times the record's weight,
+// accumulated on an interleaved complex grid.
+template <typename T, int D, int WMAX, int NC = 1>
 __global__ void __launch_bounds__(64)
     k_spread(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
              const int* __restrict__ bin_start, const int* __restrict__ cand, SpreadDev<T> a,
-             T* __restrict__ grid) {
+             T* __restrict__ grid, const T* __restrict__ str = nullptr, const uint32_t* __restrict__ sidx = nullptr) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tileb = reinterpret_cast<T*>(smem_raw);
   const int lane = threadIdx.x;
@@ -459,7 +462,7 @@ __global__ void __launch_bounds__(64)
   const int ty = a.tile[1];
   const int zstride = ty * a.sx;
   const int tile_elems = a.sx * (D >= 2 ? a.tile[1] : 1) * (D >= 3 ? a.tile[2] : 1);
-  for (int e = lane; e < tile_elems; e += 64) tileb[e] = T(0);
+  for (int e = lane; e < tile_elems * NC; e += 64) tileb[e] = T(0);
   __syncthreads();
 
   // kernel-value lane of this thread: dim kd, index kj
@@ -486,10 +489,16 @@ __global__ void __launch_bounds__(64)
           uint32_t sc = 0;
           typename Vec4<T>::type r;
           r.x = r.y = r.z = r.w = T(0);
+          T sre = T(0), sim = T(0);
           bool touch = false;
           if (j < end) {
             sc = scell[j];
             r = rec[j];
+            if (NC == 2) {
+              const uint32_t o = sidx[j];
+              sre = str[2 * (size_t)o] * r.w;
+              sim = str[2 * (size_t)o + 1] * r.w;
+            }
             touch = true;
 #pragma unroll
             for (int q = 0; q < D; ++q) {
@@ -504,7 +513,9 @@ __global__ void __launch_bounds__(64)
             const int p = __builtin_ctzll(m);
             m &= m - 1;
             const uint32_t scp = lane_read_u(sc, p);
-            const T cw = lane_read(r.w, p);
+            const T cw = NC == 2 ? T(1) : lane_read(r.w, p);
+            const T cre = NC == 2 ? lane_read(sre, p) : T(0);
+            const T cim = NC == 2 ? lane_read(sim, p) : T(0);
             const T offx = lane_read(r.x, p);
             const T offy = D >= 2 ? lane_read(r.y, p) : T(0);
             const T offz = D >= 3 ? lane_read(r.z, p) : T(0);
@@ -539,7 +550,7 @@ __global__ void __launch_bounds__(64)
               const T wx = lane_get(kv, ja < WMAX ? ja : 0);
               T v = wx * cw;
               if (D >= 2) v *= lane_get(kv, WMAX + (jb < WMAX ? jb : 0));
-              T* rowp = tileb + uy * a.sx + ux;
+              T* rowp = tileb + (uy * a.sx + ux) * NC;
               if (D == 3) {
 #pragma unroll
                 for (int jc = 0; jc < WMAX; ++jc) {
@@ -547,11 +558,23 @@ __global__ void __launch_bounds__(64)
                     int uz = uz0 + jc;
                     if (uz >= a.nf[2]) uz -= a.nf[2];
                     const T wz = lane_read(kv, 2 * WMAX + jc);
-                    if (uz < tl[2] && act) rowp[uz * zstride] += v * wz;
+                    if (uz < tl[2] && act) {
+                      if (NC == 1) {
+                        rowp[uz * zstride] += v * wz;
+                      } else {
+                        rowp[uz * zstride * 2] += v * wz * cre;
+                        rowp[uz * zstride * 2 + 1] += v * wz * cim;
+                      }
+                    }
                   }
                 }
-              } else {
-                if (act) rowp[0] += v;
+              } else if (act) {
+                if (NC == 1) {
+                  rowp[0] += v;
+                } else {
+                  rowp[0] += v * cre;
+                  rowp[1] += v * cim;
+                }
               }
             }
           }
@@ -570,7 +593,12 @@ __global__ void __launch_bounds__(64)
     int uy = rr % ny;
     int uz = rr / ny;
     int64_t gidx = ((int64_t)(t0[2] + uz) * a.nf[1] + (t0[1] + uy)) * a.nf[0] + (t0[0] + ux);
-    grid[gidx] = tileb[(uz * ty + uy) * a.sx + ux];
+    if (NC == 1) {
+      grid[gidx] = tileb[(uz * ty + uy) * a.sx + ux];
+    } else {
+      grid[2 * gidx] = tileb[2 * ((uz * ty + uy) * a.sx + ux)];
+      grid[2 * gidx + 1] = tileb[2 * ((uz * ty + uy) * a.sx + ux) + 1];
+    }
   }
 }
 
@@ -640,6 +668,44 @@ static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* gr
 
 template <typename T>
 static bool launch_spread3_reg(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s);
+
+template <typename T, int D, int WMAX>
+static void launch_spread_c(const Spreader<T>& sp, const SortedPoints<T>& p, const T* str, T* grid, hipStream_t s) {
+  SpreadDev<T> a;
+  const GridGeom& g = sp.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.maxc[q] = sp.maxc[q];
+  }
+  a.cbits = g.cell_bits;
+  a.w = sp.es.w;
+  a.sx = D >= 2 ? lds_row_stride(g.tile[0], (int)sizeof(T), sp.es.w) : g.tile[0];
+  a.cstride = sp.cand_stride;
+  a.k.beta = (T)sp.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
+  a.k.two_over_w = (T)(2.0 / sp.es.w);
+  const size_t lds = 2 * (size_t)sp.lds_bytes;
+  if (lds > 160 * 1024) throw InvalidArgument("complex spreading tile exceeds LDS");
+  hipLaunchKernelGGL((k_spread<T, D, WMAX, 2>), dim3(g.nbins), dim3(64), lds, s, p.scell.get(), p.rec.get(),
+                     p.bin_start.get(), sp.cand.get(), a, grid, str, p.idx.get());
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void Spreader<T>::spread_complex(const SortedPoints<T>& p, const Cplx<T>* str, Cplx<T>* grid, hipStream_t s) const {
+  const T* st = reinterpret_cast<const T*>(str);
+  T* gr = reinterpret_cast<T*>(grid);
+  const bool small = es.w <= 8;
+  if (geom.d == 3) {
+    small ? launch_spread_c<T, 3, 8>(*this, p, st, gr, s) : launch_spread_c<T, 3, 16>(*this, p, st, gr, s);
+  } else if (geom.d == 2) {
+    small ? launch_spread_c<T, 2, 8>(*this, p, st, gr, s) : launch_spread_c<T, 2, 16>(*this, p, st, gr, s);
+  } else {
+    launch_spread_c<T, 1, 16>(*this, p, st, gr, s);
+  }
+}
 
 template <typename T>
 void Spreader<T>::spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
@@ -1387,6 +1453,115 @@ void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const doub
 }
 
 // ---------------------------------------------------------------------------
+// complex-valued interpolation (type-2 of complex modes): one wavefront per
+// point, lanes over the (x, y) kernel pairs, loop over z, cells read from the
+// complex grid in HBM/L2; DPP reduction of the real and imaginary parts.
+// ---------------------------------------------------------------------------
+
+template <typename T, int D, int WMAX>
+__global__ void __launch_bounds__(256)
+    k_interp_c(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+               const uint32_t* __restrict__ idx, const T* __restrict__ grid, InterpDev<T> a, int64_t n,
+               double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;  // wavefront-uniform
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  constexpr int SH = Lanes<WMAX>::SH;
+  constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
+  const uint32_t sc = scell[k];
+  const typename Vec4<T>::type r = rec[k];
+  int s[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) s[q] = q < D ? (int)((sc >> (q * a.cbits)) & cmask) : 0;
+  const int kd = lane >> SH;
+  const int kj = lane & (WMAX - 1);
+  const T offk = kd == 0 ? r.x : (kd == 1 ? r.y : r.z);
+  const T kv = (kd < D && kj < w) ? es_phi<T>((T(kj) - offk) * a.k.two_over_w, a.k) : T(0);
+  T cr = T(0), ci = T(0);
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int pr = lane + 64 * ch;
+    const int ja = D == 1 ? pr : (pr & (WMAX - 1));
+    const int jb = D == 1 ? 0 : (pr >> SH);
+    const bool act = ja < w && (D == 1 || jb < w);
+    const T wx = lane_get(kv, act ? ja : 0);
+    const T wy = D >= 2 ? lane_get(kv, WMAX + (act ? jb : 0)) : T(1);
+    int gx = s[0] + ja;
+    if (gx >= a.nf[0]) gx -= a.nf[0];
+    int gy = D >= 2 ? s[1] + jb : 0;
+    if (D >= 2 && gy >= a.nf[1]) gy -= a.nf[1];
+    T sr = T(0), si = T(0);
+    if (D == 3) {
+#pragma unroll
+      for (int jc = 0; jc < WMAX; ++jc) {
+        if (jc < w) {
+          const T wz = lane_read(kv, 2 * WMAX + jc);
+          int gz = s[2] + jc;
+          if (gz >= a.nf[2]) gz -= a.nf[2];
+          if (act) {
+            const int64_t c = ((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx;
+            sr += grid[2 * c] * wz;
+            si += grid[2 * c + 1] * wz;
+          }
+        }
+      }
+    } else if (act) {
+      const int64_t c = (int64_t)gy * a.nf[0] + gx;
+      sr = grid[2 * c];
+      si = grid[2 * c + 1];
+    }
+    if (act) {
+      cr += wx * wy * sr;
+      ci += wx * wy * si;
+    }
+  }
+  cr = wave_sum63(cr);
+  ci = wave_sum63(ci);
+  if (lane == 63) {
+    const int64_t j = idx[k];
+    out[2 * j] = (double)r.w * (double)cr;
+    out[2 * j + 1] = (double)r.w * (double)ci;
+  }
+}
+
+template <typename T, int D, int WMAX>
+static void launch_interp_c(const SortedPoints<T>& p, const Cplx<T>* grid, double* out, hipStream_t s) {
+  InterpDev<T> a;
+  const GridGeom& g = p.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.dscale[q] = 0.0;
+  }
+  a.cbits = g.cell_bits;
+  a.w = p.es.w;
+  a.rx = a.ry = a.rz = a.srx = 0;
+  a.ostride = 2;
+  a.ocomp = 0;
+  a.k.beta = (T)p.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
+  a.k.two_over_w = (T)(2.0 / p.es.w);
+  hipLaunchKernelGGL((k_interp_c<T, D, WMAX>), dim3(blocks_for(p.n, 4)), dim3(256), 0, s, p.scell.get(), p.rec.get(),
+                     p.idx.get(), reinterpret_cast<const T*>(grid), a, p.n, out);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void interpolate_complex(const SortedPoints<T>& p, const Cplx<T>* grid, double* out, hipStream_t s) {
+  const bool small = p.es.w <= 8;
+  if (p.geom.d == 3) {
+    small ? launch_interp_c<T, 3, 8>(p, grid, out, s) : launch_interp_c<T, 3, 16>(p, grid, out, s);
+  } else if (p.geom.d == 2) {
+    small ? launch_interp_c<T, 2, 8>(p, grid, out, s) : launch_interp_c<T, 2, 16>(p, grid, out, s);
+  } else {
+    launch_interp_c<T, 1, 16>(p, grid, out, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // rocFFT
 // ---------------------------------------------------------------------------
 
@@ -1494,6 +1669,8 @@ template void interpolate<float>(const SortedPoints<float>&, const float*, bool,
                                  hipStream_t);
 template void interpolate<double>(const SortedPoints<double>&, const double*, bool, const double*, double*, int,
                                   int, hipStream_t);
+template void interpolate_complex<float>(const SortedPoints<float>&, const Cplx<float>*, double*, hipStream_t);
+template void interpolate_complex<double>(const SortedPoints<double>&, const Cplx<double>*, double*, hipStream_t);
 template void fill_zero<float>(float*, size_t, hipStream_t);
 template void fill_zero<double>(double*, size_t, hipStream_t);
 template void fill_zero<Cplx<float>>(Cplx<float>*, size_t, hipStream_t);

@@ -97,6 +97,9 @@ class Spreader {
   void init(const GridGeom& g, const EsParams& es);
   // grid: device T[ncells], fully overwritten.
   void spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
+  // complex strengths str[n] in original point order (times the record
+  // weight) onto a complex grid[ncells], fully overwritten.
+  void spread_complex(const SortedPoints<T>& p, const Cplx<T>* str, Cplx<T>* grid, hipStream_t s) const;
 
   GridGeom geom;
   EsParams es;
@@ -112,6 +115,11 @@ class Spreader {
 template <typename T>
 void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const double* dscale,
                  double* out, int ostride, int ocomp, hipStream_t s);
+
+// Complex-valued interpolation: out[idx] = c * sum_l b_l psi(l - g) (complex,
+// float64 interleaved, original point order).
+template <typename T>
+void interpolate_complex(const SortedPoints<T>& p, const Cplx<T>* grid, double* out, hipStream_t s);
 
 // rocFFT plan for one transform on a d-dimensional grid (lengths fastest first).
 class FftPlan {

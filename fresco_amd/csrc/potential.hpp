@@ -69,4 +69,14 @@ std::unique_ptr<Potential> make_nuwu(const double* radii, int64_t n, int d, doub
                                      const double* V, int64_t m, const double* L, double eps, int precision,
                                      double sigma);
 
+// shape[d]: grid, first entry slowest (row-major, the FFTW plan of uwu.hpp:48)
+std::unique_ptr<Potential> make_uwu(const int64_t* shape, int d, const double* Sk, const double* V, int64_t m,
+                                    int precision, int exact);
+// shape[d]: mode grid, first entry fastest (the FINUFFT n_modes of uwnu.hpp:44-82)
+std::unique_ptr<Potential> make_uwnu(const int64_t* shape, int d, const double* K, int64_t nkd, const double* Sk,
+                                     const double* V, int64_t nk, const double* L, double eps, int precision,
+                                     double sigma);
+std::unique_ptr<Potential> make_nuwnu(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
+                                      const double* Sk, const double* V, int64_t m, const double* L, int precision);
+
 }  // namespace fresco_hip

@@ -1,6 +1,9 @@
 """Structure-factor potentials on the GPU (mirror of fresco.potentials)."""
 from ._base import BasePotential
 from .combine import CombinedPotential, PythonPotential, as_cpp_potential
+from .nuwnu import NUwNU
 from .nuwu import NUwU
+from .uwnu import UwNU
+from .uwu import UwU
 
-__all__ = ["BasePotential", "CombinedPotential", "PythonPotential", "as_cpp_potential", "NUwU"]
+__all__ = ["BasePotential", "CombinedPotential", "PythonPotential", "as_cpp_potential", "NUwU", "NUwNU", "UwU", "UwNU"]

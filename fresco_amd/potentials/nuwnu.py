@@ -1,0 +1,56 @@
+"""NUwNU: non-uniform points, non-uniform k vectors.
+
+Python surface of fresco/potentials/nuwnu.pyx:9-38 over the device
+implementation of source/fresco/nuwnu.hpp (evaluated here as exact sums with
+float64 phases).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+from ._base import BasePotential, precision_code
+
+
+class NUwNU(BasePotential):
+    """phi = sum_q V_q ((|rho_q|^2/N - S0_q)/S0_q)^2 with
+    rho_q = sum_j c_j exp(+i K_q . x_j), x = (p - round(p/L) + L) 2 pi / L
+    and K used as given (nuwnu.hpp:126-182).  The gradient is
+    Re(sum_q K_q,a (-4i/N) V D rho / S0 exp(-i K_q . x_j)) with no weight
+    factor (nuwnu.hpp:184-290).  Nk = len(K) / d.
+
+    ``eps`` is kept for the constructor signature; the sums are exact.
+    Keyword-only ``precision``: "fp64" (default) or "fp32" (FP32 sin/cos
+    accumulated in blocks of 256 terms, float64 across blocks).
+    """
+
+    def __init__(self, radii, K, Sk, V, L, eps=1e-6, *, precision="fp64"):
+        self.radii = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).ravel())
+        self.N = len(self.radii)
+        self.K = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel())
+        self.Sk = np.ascontiguousarray(np.asarray(Sk, dtype=np.float64).ravel())
+        self.V = np.ascontiguousarray(np.asarray(V, dtype=np.float64).ravel())
+        assert len(self.Sk) == len(self.V)
+        self.L = np.ascontiguousarray(np.asarray(L, dtype=np.float64).ravel())
+        self.ndim = len(self.L)
+        self.eps = float(eps)
+        self.precision = precision
+        code = precision_code(precision)
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(
+            lib.fresco_nuwnu_create(
+                _lib.dptr(self.radii), self.N, self.ndim, _lib.dptr(self.K), len(self.K), _lib.dptr(self.Sk),
+                _lib.dptr(self.V), len(self.V), _lib.dptr(self.L), code, ctypes.byref(h),
+            )
+        )
+        self._init_handle(h)
+
+    def __reduce__(self):
+        return (_rebuild_nuwnu, (self.radii, self.K, self.Sk, self.V, self.L, self.eps, self.precision))
+
+
+def _rebuild_nuwnu(radii, K, Sk, V, L, eps, precision):
+    return NUwNU(radii, K, Sk, V, L, eps, precision=precision)

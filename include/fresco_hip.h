@@ -78,6 +78,52 @@ int fresco_nuwu_create(const double* radii, int64_t n, int d, double K,
                        fresco_pot** out);
 
 /*
+ * UwU: grid intensities, uniform k grid (FFT only).
+ * Replaces fresco::UwU::UwU (source/fresco/uwu.hpp:34-50) as constructed by
+ * fresco/potentials/uwu.pyx:10-22.
+ *   shape[d]      grid N, first entry slowest (row-major, the FFTW plan of uwu.hpp:48)
+ *   Sk[m], V[m]   targets and weights in FFT order (the caller applies the
+ *                 wrapper's ifftshift, uwu.pyx:14-15); m = prod(shape)
+ *   precision     FRESCO_PREC_FP32_ADJOINT = FP32 transforms, otherwise FP64
+ *   exact_grad    0: the reference's gradient Re(FFT_{-1}((2/sum x) V D rho/S0))
+ *                 (uwu.hpp:152-194); 1: the exact derivative of the energy
+ *                 (factor 4 and the d(sum x) term)
+ * ndof = m.  x is the flattened grid.
+ */
+int fresco_uwu_create(const int64_t* shape, int d, const double* Sk, const double* V, int64_t m,
+                      int precision, int exact_grad, fresco_pot** out);
+
+/*
+ * UwNU: grid values as NUFFT modes, non-uniform k vectors.
+ * Replaces fresco::UwNU::UwNU (source/fresco/uwnu.hpp:34-83) as constructed by
+ * fresco/potentials/uwnu.pyx:10-22.
+ *   shape[d]      mode grid N, first entry fastest (FINUFFT n_modes, modes -N/2..)
+ *   K[nkd]        k vectors interleaved (k0x, k0y, ...), scaled by 2 pi / L
+ *   Sk[nk], V[nk] one per k vector; nkd >= nk * d
+ *   eps, sigma    NUFFT tolerance and upsampling (0 = 2.0)
+ *   precision     FRESCO_PREC_FP32_ADJOINT = FP32 grids, otherwise FP64
+ * ndof = prod(shape).
+ */
+int fresco_uwnu_create(const int64_t* shape, int d, const double* K, int64_t nkd, const double* Sk,
+                       const double* V, int64_t nk, const double* L, double eps, int precision,
+                       double sigma, fresco_pot** out);
+
+/*
+ * NUwNU: non-uniform points, non-uniform k vectors (exact sums, float64 phases).
+ * Replaces fresco::NUwNU::NUwNU (source/fresco/nuwnu.hpp:36-79) as constructed
+ * by fresco/potentials/nuwnu.pyx:10-23.
+ *   radii[n]      point radii (weights as for NUwU)
+ *   K[nkd]        k vectors interleaved, unscaled (as in the reference); Nk = nkd / d
+ *   Sk[m], V[m]   m >= Nk
+ *   precision     FRESCO_PREC_FP32_ADJOINT = FP32 sin/cos accumulation in
+ *                 blocks (float64 sums across blocks), otherwise FP64
+ * ndof = n * d.
+ */
+int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
+                        const double* Sk, const double* V, int64_t m, const double* L, int precision,
+                        fresco_pot** out);
+
+/*
  * Energy of coordinates x[n*d] (device, interleaved x0,y0,z0,x1,...).
  * Replaces fresco::BasePotential::get_energy (base_potential.hpp:28-31), i.e.
  * NUwU::get_energy (nuwu.hpp:152-208) and its siblings.
