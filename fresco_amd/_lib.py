@@ -60,6 +60,8 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_double_p, ctypes.c_void_p],
     ),
     "fresco_pot_get_rho": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fresco_pot_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fresco_pot_set_targets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_ndof": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_nmodes": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_plan_info": (

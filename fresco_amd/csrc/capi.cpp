@@ -308,6 +308,22 @@ int fresco_pot_get_rho(fresco_pot* pot, double* rho, void* stream) {
   });
 }
 
+int fresco_pot_backward(fresco_pot* pot, double* dS0, double* dV, void* stream) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!dS0 || !dV) throw fresco_hip::InvalidArgument("null pointer");
+    p->target_grads(dS0, dV, static_cast<hipStream_t>(stream));
+  });
+}
+
+int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, void* stream) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!Sk || !V) throw fresco_hip::InvalidArgument("null pointer");
+    p->set_targets(Sk, V, static_cast<hipStream_t>(stream));
+  });
+}
+
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof) {
   return guarded([&]() { *ndof = checked(pot)->ndof(); });
 }

@@ -1673,6 +1673,7 @@ template void interpolate_complex<float>(const SortedPoints<float>&, const Cplx<
 template void interpolate_complex<double>(const SortedPoints<double>&, const Cplx<double>*, double*, hipStream_t);
 template void fill_zero<float>(float*, size_t, hipStream_t);
 template void fill_zero<double>(double*, size_t, hipStream_t);
+template void fill_zero<unsigned int>(unsigned int*, size_t, hipStream_t);
 template void fill_zero<Cplx<float>>(Cplx<float>*, size_t, hipStream_t);
 template void fill_zero<Cplx<double>>(Cplx<double>*, size_t, hipStream_t);
 

@@ -143,6 +143,27 @@ __global__ void k_nuwu_rho(const Cplx<T>* __restrict__ S, RhoDev r, int64_t m, d
   rho[2 * i + 1] = rep ? -im : im;
 }
 
+// pair tables <- full-grid targets (device), for an unchanged active set
+__global__ void k_gather_targets(const double* __restrict__ Sk, const double* __restrict__ V, PairDev pd, int npairs,
+                                 double* __restrict__ vk, double* __restrict__ vm, double* __restrict__ sk,
+                                 double* __restrict__ sm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  vk[i] = V[pd.ik[i]];
+  vm[i] = V[pd.imk[i]];
+  sk[i] = Sk[pd.ik[i]];
+  sm[i] = Sk[pd.imk[i]];
+}
+
+// number of modes with V != 0 outside the active pair set
+__global__ void k_count_inactive(const double* __restrict__ V, const unsigned char* __restrict__ active, int64_t m,
+                                 unsigned int* __restrict__ count) {
+  unsigned int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    c += (V[i] != 0.0 && !active[i]) ? 1u : 0u;
+  if (c) atomicAdd(count, c);
+}
+
 int kvec_size(int64_t m, int d) {
   // calculate_Kvec (nuwu.hpp:54-79)
   if (d == 1) return (int)((m - 1) / 2);
@@ -215,6 +236,10 @@ class NUwU final : public Potential {
       psi_[a].allocate(ps.size());
       psi_[a].upload(ps.data(), ps.size());
     }
+    Sk_full_.allocate(M_);
+    V_full_.allocate(M_);
+    Sk_full_.upload(Sk, M_);
+    V_full_.upload(V, M_);
     build_pairs(Sk, V);
 
     // coordinate maps: energy path (nuwu.hpp:160) has +L, gradient path (219) not
@@ -320,6 +345,37 @@ class NUwU final : public Potential {
     *beta = es_.beta;
   }
 
+  void target_grads(double* dS0, double* dV, hipStream_t s) override {
+    if (!rho_tmp_.get()) rho_tmp_.allocate(2 * (size_t)M_);
+    get_rho(rho_tmp_.get(), s);
+    launch_target_grads(rho_tmp_.get(), Sk_full_.get(), V_full_.get(), M_, (double)N_, nullptr, dS0, dV, s);
+  }
+
+  void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_full_.get(), Sk, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FRESCO_HIP_CHECK(hipMemcpyAsync(V_full_.get(), V, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // modes switched on outside the active pair set need new pair tables
+    fill_zero(count_.get(), 1, s);
+    hipLaunchKernelGGL(k_count_inactive, dim3(blocks_for(std::min<int64_t>(M_, 1 << 20), 256)), dim3(256), 0, s,
+                       V_full_.get(), active_.get(), M_, count_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    unsigned int cnt = 0;
+    FRESCO_HIP_CHECK(hipMemcpyAsync(&cnt, count_.get(), sizeof(cnt), hipMemcpyDeviceToHost, s));
+    FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+    if (cnt) {
+      std::vector<double> hs(M_), hv(M_);
+      FRESCO_HIP_CHECK(hipMemcpy(hs.data(), Sk_full_.get(), M_ * sizeof(double), hipMemcpyDeviceToHost));
+      FRESCO_HIP_CHECK(hipMemcpy(hv.data(), V_full_.get(), M_ * sizeof(double), hipMemcpyDeviceToHost));
+      build_pairs(hs.data(), hv.data());
+      return;
+    }
+    if (npairs_ > 0) {
+      hipLaunchKernelGGL(k_gather_targets, dim3((npairs_ + 255) / 256), dim3(256), 0, s, Sk_full_.get(),
+                         V_full_.get(), pd_, npairs_, p_vk_.get(), p_vm_.get(), p_sk_.get(), p_sm_.get());
+      FRESCO_HIP_CHECK(hipGetLastError());
+    }
+  }
+
  private:
   template <int MODE>
   void launch_loss(hipStream_t s) {
@@ -369,6 +425,11 @@ class NUwU final : public Potential {
       ip.push_back(1.0 / ps);
     }
     npairs_ = (int)h.size();
+    std::vector<unsigned char> act(M_, 0);
+    for (int q = 0; q < npairs_; ++q) act[ik[q]] = act[imk[q]] = 1;
+    active_.allocate(M_);
+    active_.upload(act.data(), M_);
+    if (!count_.get()) count_.allocate(1);
     const size_t np = std::max<size_t>(npairs_, 1);
     auto up_i = [&](DeviceBuffer<int>& b, std::vector<int>& v, size_t cnt) {
       v.resize(cnt, 0);
@@ -422,6 +483,9 @@ class NUwU final : public Potential {
   PairDev pd_{};
   int npairs_ = 0, nblk_ = 1;
   DeviceBuffer<double> partial_, e_dev_;
+  DeviceBuffer<double> Sk_full_, V_full_, rho_tmp_;  // targets on the full mode grid
+  DeviceBuffer<unsigned char> active_;               // mode belongs to an active pair
+  DeviceBuffer<unsigned int> count_;
   HostScalar host_;
   CoordMap map_e_, map_g_;
   double dscale_[3] = {0, 0, 0};

@@ -28,6 +28,21 @@ class Potential {
     (void)s;
     throw RuntimeFailure("this potential does not expose rho");
   }
+  // d phi / d S0_k and d phi / d V_k at the last evaluation's rho (device,
+  // nmodes each, the potential's own mode order)
+  virtual void target_grads(double* dS0, double* dV, hipStream_t s) {
+    (void)dS0;
+    (void)dV;
+    (void)s;
+    throw RuntimeFailure("this potential has no S0 / V targets");
+  }
+  // replace the targets S0 and weights V (device, nmodes each)
+  virtual void set_targets(const double* Sk, const double* V, hipStream_t s) {
+    (void)Sk;
+    (void)V;
+    (void)s;
+    throw RuntimeFailure("this potential has no S0 / V targets");
+  }
   virtual void plan_info(int64_t* nf, int* w, double* beta, int* tile) const {
     for (int a = 0; a < 3; ++a) {
       nf[a] = 0;
@@ -57,6 +72,13 @@ class HostScalar {
  private:
   double* p_ = nullptr;
 };
+
+// Per-mode derivatives of phi = sum V D^2, D = (|rho|^2/norm - S0)[/S0 where
+// S0 != 0]:  dphi/dV = D^2;  dphi/dS0 = -2 V D |rho|^2/(norm S0^2) where
+// S0 != 0, else -2 V D.  norm is *norm_dev when given, else norm.
+// rho: complex interleaved float64 [m].
+void launch_target_grads(const double* rho, const double* Sk, const double* V, int64_t m, double norm,
+                         const double* norm_dev, double* dS0, double* dV, hipStream_t s);
 
 // Host callback: x (host, ndof) -> *energy and, if grad != NULL, grad (host).
 // Returns 0 on success.

@@ -175,6 +175,17 @@ class UwU final : public Potential {
     FRESCO_HIP_CHECK(hipGetLastError());
   }
 
+  void target_grads(double* dS0, double* dV, hipStream_t s) override {
+    if (!rho_tmp_.get()) rho_tmp_.allocate(2 * (size_t)M_);
+    get_rho(rho_tmp_.get(), s);
+    launch_target_grads(rho_tmp_.get(), Sk_.get(), V_.get(), M_, 0.0, rhotot_.get(), dS0, dV, s);
+  }
+
+  void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
  private:
   void forward(const double* x, hipStream_t s) {
     // rhotot = sum x (uwu.hpp:128-133, accumulated deterministically)
@@ -189,7 +200,7 @@ class UwU final : public Potential {
   int64_t M_ = 0;
   FftPlan fwd_, inv_;
   DeviceBuffer<Cplx<T>> c_, rho_, f_;
-  DeviceBuffer<double> Sk_, V_, epart_, cpart_, tpart_, rhotot_, e_, cterm_;
+  DeviceBuffer<double> Sk_, V_, epart_, cpart_, tpart_, rhotot_, e_, cterm_, rho_tmp_;
   int nb_ = 1;
   HostScalar host_;
 };
@@ -350,6 +361,15 @@ class UwNU final : public Potential {
 
   void get_rho(double* rho, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(rho, rho_.get(), 2 * nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  void target_grads(double* dS0, double* dV, hipStream_t s) override {
+    launch_target_grads(rho_.get(), Sk_.get(), V_.get(), nk_, 1.0, nullptr, dS0, dV, s);
+  }
+
+  void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
   void plan_info(int64_t* nf, int* w, double* beta, int* tile) const override {
@@ -570,6 +590,15 @@ class NUwNU final : public Potential {
 
   void get_rho(double* rho, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(rho, rho_.get(), 2 * nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  void target_grads(double* dS0, double* dV, hipStream_t s) override {
+    launch_target_grads(rho_.get(), Sk_.get(), V_.get(), nk_, (double)N_, nullptr, dS0, dV, s);
+  }
+
+  void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
  private:

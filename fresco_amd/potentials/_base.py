@@ -116,6 +116,67 @@ class BasePotential:
         a = out.cpu().numpy()
         return a[0::2] + 1j * a[1::2]
 
+    # --- targets and their derivatives (differentiable S(k) shaping) -------
+
+    def _targets_flat(self, S0, V):
+        """(S0, V) in the flat mode order the library holds them in."""
+        return S0.reshape(-1), V.reshape(-1)
+
+    def _grads_user(self, dS0, dV, like):
+        """Flat library-order target gradients -> the layout of ``like``
+        (entries beyond the potential's modes, which it never reads, get 0)."""
+        n = like.numel()
+        if n != dS0.numel():
+            dS0 = torch.cat([dS0, dS0.new_zeros(n - dS0.numel())])
+            dV = torch.cat([dV, dV.new_zeros(n - dV.numel())])
+        return dS0.reshape(like.shape), dV.reshape(like.shape)
+
+    def x_grad_scale(self):
+        """Factor turning the returned gradient into d phi / d x for the
+        coordinates passed in (the reference returns d phi / d(2 pi p / L),
+        nuwu.hpp:294-314)."""
+        raise NotImplementedError
+
+    def set_targets(self, S0, V):
+        """Replace S0(k) and V(k) (same layout as the constructor's Sk / V;
+        numpy arrays or tensors)."""
+        S0 = torch.as_tensor(S0, dtype=torch.float64)
+        V = torch.as_tensor(V, dtype=torch.float64)
+        s, v = self._targets_flat(S0, V)
+        s = s[: self._nmodes].to(device=device(), dtype=torch.float64).contiguous()
+        v = v[: self._nmodes].to(device=device(), dtype=torch.float64).contiguous()
+        if s.numel() != self._nmodes or v.numel() != self._nmodes:
+            raise ValueError("S0 / V must hold %d modes" % self._nmodes)
+        _lib.check(
+            _lib.load().fresco_pot_set_targets(
+                self._h, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(v.data_ptr()), stream_handle()
+            )
+        )
+        self._dev_targets = (s, v)
+
+    def targets(self):
+        """Current (S0, V) in the constructor's layout (numpy)."""
+        return self._current("Sk"), self._current("V")
+
+    def _current(self, name):
+        """Constructor-order Sk / V as numpy, including set_targets updates."""
+        dev = getattr(self, "_dev_targets", None)
+        if dev is None:
+            return getattr(self, name)
+        return dev[0 if name == "Sk" else 1].cpu().numpy()
+
+    def target_grads(self):
+        """(d phi/d S0, d phi/d V) at the last evaluation, float64 device
+        tensors in the library's flat mode order."""
+        dS0 = torch.empty(self._nmodes, dtype=torch.float64, device=device())
+        dV = torch.empty_like(dS0)
+        _lib.check(
+            _lib.load().fresco_pot_backward(
+                self._h, ctypes.c_void_p(dS0.data_ptr()), ctypes.c_void_p(dV.data_ptr()), stream_handle()
+            )
+        )
+        return dS0, dV
+
     def plan_info(self):
         lib = _lib.load()
         nf = (ctypes.c_int64 * 3)()

@@ -48,8 +48,11 @@ class NUwNU(BasePotential):
         )
         self._init_handle(h)
 
+    def x_grad_scale(self):
+        return np.tile(2 * np.pi / self.L, self.N)
+
     def __reduce__(self):
-        return (_rebuild_nuwnu, (self.radii, self.K, self.Sk, self.V, self.L, self.eps, self.precision))
+        return (_rebuild_nuwnu, (self.radii, self.K, self._current("Sk"), self._current("V"), self.L, self.eps, self.precision))
 
 
 def _rebuild_nuwnu(radii, K, Sk, V, L, eps, precision):

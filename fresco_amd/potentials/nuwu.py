@@ -51,10 +51,13 @@ class NUwU(BasePotential):
         )
         self._init_handle(h)
 
+    def x_grad_scale(self):
+        return np.tile(2 * np.pi / self.L, self.N)
+
     def __reduce__(self):
         return (
             _rebuild_nuwu,
-            (self.radii, self.K, self.Sk, self.V, self.L, self.eps, self.precision, self.sigma),
+            (self.radii, self.K, self._current("Sk"), self._current("V"), self.L, self.eps, self.precision, self.sigma),
         )
 
 

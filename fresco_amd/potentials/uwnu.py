@@ -50,8 +50,12 @@ class UwNU(BasePotential):
         )
         self._init_handle(h)
 
+    def x_grad_scale(self):
+        # uwnu.hpp:155 divides the derivative by 2 pi
+        return np.full(int(np.prod(self.N)), 2 * np.pi)
+
     def __reduce__(self):
-        return (_rebuild_uwnu, (self.N, self.K, self.Sk, self.V, self.L, self.eps, self.precision, self.sigma))
+        return (_rebuild_uwnu, (self.N, self.K, self._current("Sk"), self._current("V"), self.L, self.eps, self.precision, self.sigma))
 
 
 def _rebuild_uwnu(N, K, Sk, V, L, eps, precision, sigma):

@@ -52,11 +52,38 @@ class UwU(BasePotential):
         )
         self._init_handle(h)
 
+    def x_grad_scale(self):
+        if not self.exact_grad:
+            raise ValueError("the reference UwU gradient is not the derivative of its energy; "
+                             "construct with exact_grad=True to differentiate through it")
+        return np.ones(int(np.prod(self.N)))
+
+    def targets(self):
+        shape = tuple(int(v) for v in self.N)
+        return (np.fft.fftshift(self._current("Sk").reshape(shape)),
+                np.fft.fftshift(self._current("V").reshape(shape)))
+
+    def _targets_flat(self, S0, V):
+        # centred grids -> FFT order, as the constructor (uwu.pyx:14-15)
+        import torch
+
+        return torch.fft.ifftshift(S0).reshape(-1), torch.fft.ifftshift(V).reshape(-1)
+
+    def _grads_user(self, dS0, dV, like):
+        import torch
+
+        shape = tuple(int(v) for v in self.N)
+        if like.dim() == 1:
+            # a flat array was shifted as 1D by the constructor's ifftshift
+            return torch.fft.fftshift(dS0), torch.fft.fftshift(dV)
+        return torch.fft.fftshift(dS0.reshape(shape)).reshape(like.shape), torch.fft.fftshift(
+            dV.reshape(shape)).reshape(like.shape)
+
     def __reduce__(self):
         # Sk / V are stored shifted; undo the shift so the constructor reapplies it
         shape = tuple(int(v) for v in self.N)
-        Sk = np.fft.fftshift(self.Sk.reshape(shape))
-        V = np.fft.fftshift(self.V.reshape(shape))
+        Sk = np.fft.fftshift(self._current("Sk").reshape(shape))
+        V = np.fft.fftshift(self._current("V").reshape(shape))
         return (_rebuild_uwu, (self.N, self.K, Sk, V, self.L, self.eps, self.precision, self.exact_grad))
 
 

@@ -148,6 +148,27 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad,
  */
 int fresco_pot_get_rho(fresco_pot* pot, double* rho, void* stream);
 
+/*
+ * Derivatives of the last evaluation's energy with respect to the targets:
+ * dS0[m], dV[m] (device, float64, the potential's mode order; m = nmodes):
+ *   dphi/dV_k  = D_k^2
+ *   dphi/dS0_k = -2 V_k D_k s_k / S0_k^2   (S0_k != 0),   -2 V_k D_k   (S0_k == 0)
+ * with s_k = |rho_k|^2 / norm and D_k as in the loss (nuwu.hpp:260-276).
+ * The backward of the autograd function; no reference counterpart (the
+ * reference's S0 / V are constructor constants, nuwu.hpp:25-26).
+ */
+int fresco_pot_backward(fresco_pot* pot, double* dS0, double* dV, void* stream);
+
+/*
+ * Replace the targets S0 and weights V (device, float64, nmodes each, the
+ * order the constructor takes them in after the wrapper's preprocessing).
+ * No reference counterpart (constructor constants there).  For NUwU a mode
+ * switched on outside the constructor's active set (V_k = V_-k = 0 before)
+ * rebuilds the mode-pair tables on the host; otherwise the update stays on
+ * the device.  Synchronises the stream once.
+ */
+int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, void* stream);
+
 /* Number of degrees of freedom the potential expects (n*d for points, prod(N) for grids). */
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof);
 

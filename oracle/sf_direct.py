@@ -222,6 +222,21 @@ def _loss(rho, Sk, V, norm):
     return phi, g
 
 
+def target_grads(rho, Sk, V, norm):
+    """d phi / d S0 and d phi / d V of phi = sum V D^2 (the loss of
+    nuwu.hpp:260-276 as a function of its targets): dphi/dV = D^2;
+    dphi/dS0 = -2 V D s / S0^2 where S0 != 0 (D = s/S0 - 1), else -2 V D
+    (D = s - S0), s = |rho|^2 / norm."""
+    Sk = np.asarray(Sk, dtype=np.float64)
+    V = np.asarray(V, dtype=np.float64)
+    s = np.abs(rho) ** 2 / norm
+    nz = Sk != 0
+    Ssafe = np.where(nz, Sk, 1.0)
+    D = np.where(nz, (s - Sk) / Ssafe, s - Sk)
+    dS0 = np.where(nz, -2.0 * V * D * s / Ssafe ** 2, -2.0 * V * D)
+    return dS0, D * D
+
+
 # ---------------------------------------------------------------------------
 # NUwU  (source/fresco/nuwu.hpp)
 # ---------------------------------------------------------------------------

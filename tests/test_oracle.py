@@ -249,3 +249,29 @@ def test_uwnu_gradient_is_fd_over_two_pi():
 def test_uwnu_mode_layout_first_dimension_fastest():
     m = O.uwnu_mode_index((4, 3))
     assert m[:5].tolist() == [[-2, -1], [-1, -1], [0, -1], [1, -1], [-2, 0]]
+
+
+def test_target_grads_fd():
+    # d phi / d S0 and d phi / d V by central differences of the loss itself
+    rng = np.random.default_rng(12)
+    m = 40
+    rho = rng.normal(size=m) + 1j * rng.normal(size=m)
+    Sk = rng.uniform(0.3, 2.0, m)
+    Sk[::4] = 0.0
+    V = rng.uniform(0, 1, m)
+    norm = 3.0
+    dS0, dV = O.target_grads(rho, Sk, V, norm)
+    h = 1e-6
+    for i in range(m):
+        e = np.zeros(m)
+        e[i] = h
+        fv = (O._loss(rho, Sk, V + e, norm)[0] - O._loss(rho, Sk, V - e, norm)[0]) / (2 * h)
+        assert abs(fv - dV[i]) <= 1e-6 * max(1.0, abs(dV[i]))
+        if Sk[i] != 0:
+            fs = (O._loss(rho, Sk + e, V, norm)[0] - O._loss(rho, Sk - e, V, norm)[0]) / (2 * h)
+            assert abs(fs - dS0[i]) <= 1e-5 * max(1.0, abs(dS0[i]))
+        else:
+            # one-sided: the S0 == 0 branch (D = s - S0) is only taken at 0;
+            # its derivative is that of (s - S0)^2 V
+            s_i = abs(rho[i]) ** 2 / norm
+            assert dS0[i] == pytest.approx(-2 * V[i] * s_i)
