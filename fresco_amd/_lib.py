@@ -40,6 +40,11 @@ SIGNATURES = {
         [c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double, c_double_p, c_double_p,
          ctypes.c_int64, c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, c_void_pp],
     ),
+    "fresco_pair_create": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_double, ctypes.c_double, c_double_p, ctypes.c_int64, ctypes.c_int, c_double_p,
+         ctypes.c_int, ctypes.c_int, ctypes.c_double, c_void_pp],
+    ),
     "fresco_uwu_create": (
         ctypes.c_int,
         [c_int64_p, ctypes.c_int, c_double_p, c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_pp],

@@ -141,6 +141,18 @@ int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, 
   });
 }
 
+int fresco_pair_create(int kind, double p1, double eps, const double* radii, int64_t n, int d, const double* box,
+                       int method, int use_cell_lists, double ncellx_scale, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!radii) throw fresco_hip::InvalidArgument("null array argument");
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_pair(kind, p1, eps, radii, n, d, box, method, use_cell_lists, ncellx_scale);
+    *out = h.release();
+  });
+}
+
 int fresco_combined_create(fresco_pot* const* pots, int count, fresco_pot** out) {
   return guarded([&]() {
     if (!out) throw fresco_hip::InvalidArgument("out is null");

@@ -101,4 +101,9 @@ std::unique_ptr<Potential> make_uwnu(const int64_t* shape, int d, const double* 
 std::unique_ptr<Potential> make_nuwnu(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
                                       const double* Sk, const double* V, int64_t m, const double* L, int precision);
 
+// kind 0 inverse power (p1 = exponent a), 1 WCA (p1 = sigma); method 1
+// cartesian, 2 periodic (box[d]); cells != 0: cell lists (periodic only)
+std::unique_ptr<Potential> make_pair(int kind, double p1, double eps, const double* radii, int64_t n, int d,
+                                     const double* box, int method, int cells, double ncellx_scale);
+
 }  // namespace fresco_hip

@@ -123,6 +123,32 @@ int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, 
                         const double* Sk, const double* V, int64_t m, const double* L, int precision,
                         fresco_pot** out);
 
+/* pair interactions and distance methods for fresco_pair_create
+ * (method values = fresco.distances.Distance, distance_enum.py) */
+#define FRESCO_PAIR_INVERSE_POWER 0
+#define FRESCO_PAIR_WCA 1
+#define FRESCO_DIST_CARTESIAN 1
+#define FRESCO_DIST_PERIODIC 2
+
+/*
+ * Pair potential of n particles in d = 1..4 dimensions.
+ * Replaces InversePowerCartesian / InversePowerPeriodic /
+ * InversePowerPeriodicCellLists (source/fresco/inversepower_potential.hpp:81-121)
+ * and WCA / WCAPeriodic / WCAPeriodicCellLists (source/fresco/wca.hpp:85-135)
+ * as constructed by fresco/potentials/inversepower_potential.pyx:15-83 and
+ * wca_potential.pyx:14-60.
+ *   kind            FRESCO_PAIR_INVERSE_POWER: E = eps/a (1 - r/(r_i+r_j))^a for r < r_i+r_j,
+ *                   p1 = a;  FRESCO_PAIR_WCA: p1 = sigma (radii unused by the energy)
+ *   box[d]          periodic box (may be NULL for FRESCO_DIST_CARTESIAN)
+ *   use_cell_lists  periodic only; cell sides are at least the interaction range
+ *                   (also for WCA, whose cutoff 2^(1/6) sigma the reference's
+ *                   rcut = 2 max(r) can miss); ncellx_scale is accepted and unused
+ * ndof = n * d.  The gradient is dE/dx (simple_pairwise_potential.hpp:99-129).
+ */
+int fresco_pair_create(int kind, double p1, double eps, const double* radii, int64_t n, int d,
+                       const double* box, int method, int use_cell_lists, double ncellx_scale,
+                       fresco_pot** out);
+
 /*
  * Energy of coordinates x[n*d] (device, interleaved x0,y0,z0,x1,...).
  * Replaces fresco::BasePotential::get_energy (base_potential.hpp:28-31), i.e.
