@@ -642,6 +642,12 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
   cand.upload(host.data(), host.size());
   lds_bytes = (int)spread_lds_bytes(g, (int)sizeof(T), e.w);
   if (lds_bytes > 160 * 1024) throw InvalidArgument("spreader tile exceeds LDS");
+  // register-tile spreader on a grid with few tiles: slice each tile's
+  // points over several wavefronts (about 8 per CU in total)
+  parts = 1;
+  if (g.d == 3 && g.tile[0] == 16 && g.tile[1] == 16 && g.tile[2] == 8 && g.nf[1] >= 16 + e.w - 1)
+    parts = std::max(1, std::min(16, (8192 + g.nbins - 1) / g.nbins));
+  if (parts > 1) part_buf.allocate((size_t)parts * (size_t)g.ncells());
 }
 
 template <typename T, int D, int WMAX>
@@ -859,9 +865,23 @@ __global__ void __launch_bounds__(256)
 // every cell sums its points in one fixed order (bitwise reproducible), and
 // is written to HBM once.  Requires nf[1] >= 16 + w - 1 (so the tile sees
 // one contiguous piece of any support along y).
+// Small grids have few tiles: gridDim.y > 1 splits every candidate bin's
+// point range into gridDim.y contiguous slices, wavefront (tile, part) writes
+// its partial tile to grid + part * ncells, and k_sum_parts adds the parts
+// in a fixed order (still bitwise reproducible).
 // ---------------------------------------------------------------------------
 
 constexpr int RT_X = 16, RT_Y = 16, RT_Z = 8;
+
+// grid[i] = sum_{p < parts} part[p * n + i], in order p = 0, 1, ...
+template <typename T>
+__global__ void k_sum_parts(const T* __restrict__ part, int parts, int64_t n, T* __restrict__ grid) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T acc = part[i];
+    for (int p = 1; p < parts; ++p) acc += part[(int64_t)p * n + i];
+    grid[i] = acc;
+  }
+}
 
 template <typename T, int WMAX>
 __global__ void __launch_bounds__(64)
@@ -911,8 +931,10 @@ __global__ void __launch_bounds__(64)
         const int bx = cand[t[0] * a.maxc[0] + c0];
         if (bx < 0) continue;
         const int bin = bx + a.ntile[0] * (by + a.ntile[1] * bz);
-        const int beg = bin_start[bin];
-        const int end = bin_start[bin + 1];
+        const int beg0 = bin_start[bin];
+        const int len = bin_start[bin + 1] - beg0;
+        const int beg = beg0 + (int)(((int64_t)len * blockIdx.y) / gridDim.y);
+        const int end = beg0 + (int)(((int64_t)len * (blockIdx.y + 1)) / gridDim.y);
         for (int base = beg; base < end; base += 64) {
           const int j = base + lane;
           uint32_t sc = 0;
@@ -994,6 +1016,7 @@ __global__ void __launch_bounds__(64)
       }
     }
   }
+  if (gridDim.y > 1) grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1] * a.nf[2];
   if (x_l < tl[0]) {
     const int64_t gx = t0[0] + x_l;
 #pragma unroll
@@ -1138,9 +1161,17 @@ static void launch_spread3_reg_w(const Spreader<T>& sp, const SortedPoints<T>& p
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
-  hipLaunchKernelGGL((k_spread3_reg<T, WMAX>), dim3(g.nbins), dim3(64), 0, s, p.scell.get(), p.rec.get(),
-                     p.bin_start.get(), sp.cand.get(), a, grid);
+  const int parts = sp.parts;
+  T* out = parts > 1 ? sp.part_buf.get() : grid;
+  hipLaunchKernelGGL((k_spread3_reg<T, WMAX>), dim3(g.nbins, parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
+                     p.bin_start.get(), sp.cand.get(), a, out);
   FRESCO_HIP_CHECK(hipGetLastError());
+  if (parts > 1) {
+    const size_t nc4 = (size_t)g.ncells();
+    hipLaunchKernelGGL(k_sum_parts<T>, dim3((unsigned)std::min<size_t>(blocks_for(nc4, 256), 16384)), dim3(256), 0,
+                       s, out, parts, (int64_t)nc4, grid);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
 }
 
 template <typename T>
@@ -1250,8 +1281,12 @@ __global__ void __launch_bounds__(G8_THREADS)
   const int w = a.w;
   const uint32_t cmask = (1u << a.cbits) - 1u;
   const int b = blockIdx.x;
-  const int beg = bin_start[b];
-  const int end = bin_start[b + 1];
+  // gridDim.y > 1: the bin's points are split into contiguous slices, one per
+  // workgroup (each stages the same region)
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.y) / gridDim.y);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.y + 1)) / gridDim.y);
   if (beg == end) return;
   int t0[3];
   t0[0] = (b % a.ntile[0]) * a.tile[0];
@@ -1383,8 +1418,10 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   a.k.two_over_w = (T)(2.0 / p.es.w);
   size_t lds = sizeof(T) * (size_t)a.rx * a.ry * a.rz;
   if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS");
-  hipLaunchKernelGGL((k_interp3_g8<T, DERIV>), dim3(g.nbins), dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
-                     p.idx.get(), p.bin_start.get(), grid, a, out);
+  // enough workgroups to fill the GPU: split bins when there are few
+  const int chunks = std::max(1, std::min(8, (2048 + g.nbins - 1) / g.nbins));
+  hipLaunchKernelGGL((k_interp3_g8<T, DERIV>), dim3(g.nbins, chunks), dim3(G8_THREADS), lds, s, p.scell.get(),
+                     p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 

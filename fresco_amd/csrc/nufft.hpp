@@ -107,6 +107,8 @@ class Spreader {
   DeviceBuffer<int> cand;  // [3][maxtile][MAXC] candidate bins per tile, -1 padded
   int cand_stride = 0;     // entries per dimension block
   int lds_bytes = 0;
+  int parts = 1;             // register-tile spreader: point-range slices per tile
+  DeviceBuffer<T> part_buf;  // [parts][ncells] partial grids when parts > 1
 };
 
 // Interpolation of a real fine grid at the sorted points.

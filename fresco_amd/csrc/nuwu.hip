@@ -175,8 +175,9 @@ int kvec_size(int64_t m, int d) {
 template <typename T>
 class NUwU final : public Potential {
  public:
+  // force_band >= 0: fine grid for modes |k_a| <= force_band regardless of V
   NUwU(const double* radii, int64_t n, int d, double K, const double* Sk, const double* V, int64_t m,
-       const double* L, double eps, int precision, double sigma)
+       const double* L, double eps, int precision, double sigma, int force_band = -1)
       : N_(n), d_(d), plain_(precision == 0) {
     (void)K;  // stored but unused by the arithmetic (nuwu.hpp:23,39)
     if (d < 1 || d > 3) throw InvalidArgument("Invalid dimension: " + std::to_string(d));
@@ -195,12 +196,12 @@ class NUwU final : public Potential {
       L_[a] = L[a];
     }
     if (sigma <= 0) sigma = 2.0;
+    sigma_ = sigma;
+    eps_ = eps;
+    precision_ = precision;
     // the adjoint path interpolates the kernel derivative: one extra width
     // (two in 1D) keeps its gradient at the accuracy the eps rule gives values
     es_ = es_params(eps, sigma, plain_ ? 0 : (d == 1 ? 2 : 1));
-    int nf[3] = {1, 1, 1};
-    for (int a = 0; a < d; ++a) nf[a] = next_smooth_even((int)std::ceil(sigma * nm_), es_.w);
-    geom_ = make_geom(d, nf, es_.w, (int)sizeof(T));
 
     // weights c (nuwu.hpp:125-149)
     std::vector<double> c(n, 1.0);
@@ -217,29 +218,13 @@ class NUwU final : public Potential {
     c_.allocate(n);
     c_.upload(ct.data(), n);
 
-    pts_.init(geom_, es_, n);
-    spreader_.init(geom_, es_);
-    grid_.allocate(geom_.ncells());
-    hx_ = geom_.nf[0] / 2 + 1;
-    nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
-    S_.allocate(nspec_);
-    nX_ = plain_ ? d : 1;
-    for (int a = 0; a < nX_; ++a) X_[a].allocate(nspec_);
-    const bool dbl = sizeof(T) == 8;
-    r2c_.create(FftPlan::R2C, dbl, d, geom_.nf);
-    c2r_.create(FftPlan::C2R, dbl, d, geom_.nf);
-
-    // deconvolution factors per dimension
-    for (int a = 0; a < 3; ++a) {
-      std::vector<double> ps = a < d ? es_deconv(es_, geom_.nf[a], kp_) : std::vector<double>(kp_ + 1, 1.0);
-      psi_host_[a] = ps;
-      psi_[a].allocate(ps.size());
-      psi_[a].upload(ps.data(), ps.size());
-    }
+    radii_host_.assign(radii, radii + n);
+    K_ = K;
     Sk_full_.allocate(M_);
     V_full_.allocate(M_);
     Sk_full_.upload(Sk, M_);
     V_full_.upload(V, M_);
+    setup_grid(force_band >= 0 ? force_band : active_band(V));
     build_pairs(Sk, V);
 
     // coordinate maps: energy path (nuwu.hpp:160) has +L, gradient path (219) not
@@ -249,7 +234,6 @@ class NUwU final : public Potential {
       map_e_.shift[a] = La;
       map_g_.shift[a] = 0.0;
       map_e_.scale[a] = map_g_.scale[a] = 2.0 * M_PI / La;
-      dscale_[a] = a < d ? -(2.0 / es_.w) * (geom_.nf[a] / (2.0 * M_PI)) : 0.0;
     }
   }
 
@@ -257,6 +241,7 @@ class NUwU final : public Potential {
   int64_t nmodes() const override { return M_; }
 
   double energy(const double* x, hipStream_t s) override {
+    last_x_ = x;
     {
       ProfScope p(ST_SORT, s);
       pts_.set(x, map_e_, c_.get(), s);
@@ -278,6 +263,7 @@ class NUwU final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    last_x_ = x;
     {
       ProfScope p(ST_SORT, s);
       pts_.set(x, map_g_, c_.get(), s);
@@ -323,6 +309,19 @@ class NUwU final : public Potential {
   }
 
   void get_rho(double* rho, hipStream_t s) override {
+    if (band_ < kp_) {
+      // the fine grid covers only the active band: evaluate the full mode
+      // grid from the last coordinates
+      if (!last_x_) throw RuntimeFailure("get_rho needs a previous evaluation");
+      if (!full_) {
+        const std::vector<double> zero(M_, 0.0);
+        full_ = std::make_unique<NUwU<T>>(radii_host_.data(), N_, d_, K_, zero.data(), zero.data(), M_, L_, eps_,
+                                          precision_, sigma_, kp_);
+      }
+      full_->energy(last_x_, s);
+      full_->get_rho(rho, s);
+      return;
+    }
     RhoDev r;
     r.d = d_;
     r.kp = kp_;
@@ -366,6 +365,8 @@ class NUwU final : public Potential {
       std::vector<double> hs(M_), hv(M_);
       FRESCO_HIP_CHECK(hipMemcpy(hs.data(), Sk_full_.get(), M_ * sizeof(double), hipMemcpyDeviceToHost));
       FRESCO_HIP_CHECK(hipMemcpy(hv.data(), V_full_.get(), M_ * sizeof(double), hipMemcpyDeviceToHost));
+      const int nb = active_band(hv.data());
+      if (nb > band_) setup_grid(nb);
       build_pairs(hs.data(), hv.data());
       return;
     }
@@ -377,6 +378,51 @@ class NUwU final : public Potential {
   }
 
  private:
+  // largest |k_a| over modes whose pair {k, -k} carries weight; the FP64
+  // plain path keeps the reference's full mode grid
+  int active_band(const double* V) const {
+    if (plain_ || !V) return kp_;
+    int b = 0;
+    const int n = nm_;
+    for (int64_t i = 0; i < M_; ++i) {
+      if (V[i] == 0.0) continue;
+      int k[3] = {0, 0, 0};
+      k[0] = (int)(i % n) - kp_;
+      if (d_ >= 2) k[1] = (int)((i / n) % n) - kp_;
+      if (d_ >= 3) k[2] = (int)(i / ((int64_t)n * n)) - kp_;
+      for (int a = 0; a < d_; ++a) b = std::max(b, std::abs(k[a]));
+    }
+    return b;
+  }
+
+  // fine grid sized for the modes |k_a| <= band (sigma (2 band + 1) cells, the
+  // NUFFT's accuracy for those modes is that of the full grid)
+  void setup_grid(int band) {
+    band_ = band;
+    const int nmb = 2 * band + 1;
+    int nf[3] = {1, 1, 1};
+    for (int a = 0; a < d_; ++a) nf[a] = next_smooth_even((int)std::ceil(sigma_ * nmb), es_.w);
+    geom_ = make_geom(d_, nf, es_.w, (int)sizeof(T));
+    pts_.init(geom_, es_, N_);
+    spreader_.init(geom_, es_);
+    grid_.allocate(geom_.ncells());
+    hx_ = geom_.nf[0] / 2 + 1;
+    nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
+    S_.allocate(nspec_);
+    nX_ = plain_ ? d_ : 1;
+    for (int a = 0; a < nX_; ++a) X_[a].allocate(nspec_);
+    const bool dbl = sizeof(T) == 8;
+    r2c_.create(FftPlan::R2C, dbl, d_, geom_.nf);
+    c2r_.create(FftPlan::C2R, dbl, d_, geom_.nf);
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> ps = a < d_ ? es_deconv(es_, geom_.nf[a], kp_) : std::vector<double>(kp_ + 1, 1.0);
+      psi_host_[a] = ps;
+      psi_[a].allocate(ps.size());
+      psi_[a].upload(ps.data(), ps.size());
+    }
+    for (int a = 0; a < 3; ++a) dscale_[a] = a < d_ ? -(2.0 / es_.w) * (geom_.nf[a] / (2.0 * M_PI)) : 0.0;
+  }
+
   template <int MODE>
   void launch_loss(hipStream_t s) {
     Cplx<T>* X0 = X_[0].get();
@@ -489,6 +535,12 @@ class NUwU final : public Potential {
   HostScalar host_;
   CoordMap map_e_, map_g_;
   double dscale_[3] = {0, 0, 0};
+  double sigma_ = 2.0, eps_ = 1e-6, K_ = 0.0;
+  int precision_ = 0;
+  int band_ = 0;                          // modes |k_a| <= band_ are on the fine grid
+  std::vector<double> radii_host_;
+  const double* last_x_ = nullptr;        // coordinates of the last evaluation
+  std::unique_ptr<NUwU<T>> full_;         // full-mode-grid evaluator for get_rho
 };
 
 }  // namespace

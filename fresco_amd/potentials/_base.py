@@ -69,6 +69,7 @@ class BasePotential:
     def get_energy(self, x):
         lib = _lib.load()
         xt, _ = as_device_f64(x, self._ndof)
+        self._last_x = xt  # get_rho / target_grads may re-read the last coordinates
         e = ctypes.c_double()
         _lib.check(lib.fresco_pot_energy(self._h, ctypes.c_void_p(xt.data_ptr()), ctypes.byref(e), stream_handle()))
         return float(e.value)
@@ -76,6 +77,7 @@ class BasePotential:
     def get_energy_gradient(self, x):
         lib = _lib.load()
         xt, was_torch = as_device_f64(x, self._ndof)
+        self._last_x = xt
         g = torch.empty_like(xt)
         e = ctypes.c_double()
         _lib.check(
@@ -98,6 +100,7 @@ class BasePotential:
     def energy_gradient_device(self, x, grad, stream=None):
         """Zero-copy call: x, grad are contiguous float64 device tensors."""
         lib = _lib.load()
+        self._last_x = x
         e = ctypes.c_double()
         s = stream_handle() if stream is None else ctypes.c_void_p(stream.cuda_stream)
         _lib.check(

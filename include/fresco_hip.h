@@ -171,6 +171,9 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad,
  * reference would compute from it is |rho_k|^2 / N (nuwu.hpp:262-264).
  * No reference counterpart: exposes an intermediate of nuwu.hpp:258 for the
  * final-S(k) report and the autograd backward.
+ * When the fine grid only covers the active band (NUwU adjoint precisions with
+ * V = 0 outside |k_a| <= band), this re-reads the last evaluation's x, which
+ * must still hold those coordinates; fresco_pot_backward does the same.
  */
 int fresco_pot_get_rho(fresco_pot* pot, double* rho, void* stream);
 

@@ -41,8 +41,10 @@ def NUwU():
     return cls
 
 
+# (3, 21, 3000): a fine grid (42-48 cells) with few register-spreader tiles,
+# so each tile's points are sliced over several wavefronts
 @pytest.mark.parametrize("prec", ["fp64", "fp64-adjoint", "fp32"])
-@pytest.mark.parametrize("d,n,N", [(1, 33, 200), (2, 17, 400), (3, 9, 300)])
+@pytest.mark.parametrize("d,n,N", [(1, 33, 200), (2, 17, 400), (3, 9, 300), (3, 21, 3000)])
 def test_parity_vs_direct_sum(NUwU, prec, d, n, N):
     radii, Sk, V, p = make_case(d, n, N, seed=10 * d + n)
     L = np.ones(d)
@@ -176,3 +178,39 @@ def test_c2_size_fp32_vs_fp64_and_invariants(NUwU):
     assert gs < 1e-8 * float(g64.abs().max()) * N ** 0.5
     Es = p64.get_energy(xt + 0.123456)
     assert abs(Es - E64) / abs(E64) < 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp64-adjoint", "fp32"])
+def test_active_band_grid(NUwU, prec):
+    """V = 0 outside |k| <= 4 on a 21^3 mode grid: the adjoint paths size the
+    fine grid by the active band; energy / gradient / full-grid rho and the
+    target derivatives still match the exact sums, and switching on modes
+    beyond the band rebuilds the grid."""
+    rng = np.random.default_rng(31)
+    d, n, N = 3, 21, 2000
+    k = O.mode_grid(10, 3)
+    kk = np.linalg.norm(k, axis=1)
+    V = ((kk > 0) & (kk <= 4)).astype(float)
+    Sk = rng.uniform(0.5, 2.0, n ** d)
+    p = rng.uniform(0, 1, N * d)
+    radii = rng.uniform(0.8, 1.2, N)
+    pot = NUwU(radii, 0, Sk, V, np.ones(d), EPS[prec], precision=prec)
+    full = NUwU(radii, 0, Sk, V, np.ones(d), EPS[prec], precision="fp64")
+    assert pot.plan_info()["nf"][0] < full.plan_info()["nf"][0]
+    ref = O.NUwU(radii, 0, Sk, V, np.ones(d))
+    E0, g0 = ref.get_energy_gradient(p)
+    E, g = pot.get_energy_gradient(p)
+    assert abs(E - E0) / abs(E0) < TOL[prec]
+    assert rel(g, g0) < TOL[prec]
+    assert rel(pot.get_rho(), ref.rho(p)) < TOL[prec]
+    dS0, dV = pot.target_grads()
+    eS0, eV = O.target_grads(ref.rho(p), Sk, V, N)
+    assert rel(dS0.cpu().numpy(), eS0) < TOL[prec]
+    assert rel(dV.cpu().numpy(), eV) < TOL[prec]
+    V2 = ((kk > 0) & (kk <= 7)).astype(float)
+    pot.set_targets(Sk, V2)
+    E2, g2 = pot.get_energy_gradient(p)
+    r2 = O.NUwU(radii, 0, Sk, V2, np.ones(d))
+    E20, g20 = r2.get_energy_gradient(p)
+    assert abs(E2 - E20) / abs(E20) < TOL[prec]
+    assert rel(g2, g20) < TOL[prec]
