@@ -109,6 +109,13 @@ class Spreader {
   int lds_bytes = 0;
   int parts = 1;             // register-tile spreader: point-range slices per tile
   DeviceBuffer<T> part_buf;  // [parts][ncells] partial grids when parts > 1
+  // input-driven halo spreader (3D, w <= 8)
+  enum Mode { REG = 0, HALO = 1 };
+  int mode = REG;
+  bool halo_ok = false;
+  int halo_parts = 1;          // point-range slices per bin
+  DeviceBuffer<T> halo_buf;    // [nbins][halo_parts][halo tile]
+  DeviceBuffer<int> halo_tab;  // per-dimension coverage tables for the merge
 };
 
 // Interpolation of a real fine grid at the sorted points.

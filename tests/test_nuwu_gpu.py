@@ -214,3 +214,21 @@ def test_active_band_grid(NUwU, prec):
     E20, g20 = r2.get_energy_gradient(p)
     assert abs(E2 - E20) / abs(E20) < TOL[prec]
     assert rel(g2, g20) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec,eps,tol", [("fp32", 1e-6, 1e-5), ("fp64", 1e-7, 1e-6)])
+def test_halo_spreader_parity_and_determinism(NUwU, prec, eps, tol):
+    """3D grids of few bins with a kernel at most 8 wide take the input-driven
+    halo spreader (LDS tile plus one-sided halo, fixed-order merge): parity
+    with the exact sums and bitwise-equal repeated evaluations."""
+    radii, Sk, V, p = make_case(3, 21, 5000, seed=41)
+    pot = NUwU(radii, 0.0, Sk, V, np.ones(3), eps, precision=prec)
+    info = pot.plan_info()
+    assert info["w"] <= 8 and min(info["nf"]) >= 23
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
+    E0, g0 = ref.get_energy_gradient(p)
+    E, g = pot.get_energy_gradient(p)
+    assert abs(E - E0) / abs(E0) < tol
+    assert rel(g, g0) < tol
+    E2, g2 = pot.get_energy_gradient(p)
+    assert E == E2 and np.array_equal(g, g2)
