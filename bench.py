@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (the HBM byte-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box in this setup has one GPU; "
@@ -110,8 +112,18 @@ def main():
 
     e32 = pot32.energy_gradient_device(x, g32)
     e64 = pot64.energy_gradient_device(x, g64)
+    info32 = pot32.plan_info()  # after the first evaluation: the autotuned spreader
     rel_e = abs(e32 - e64) / abs(e64)
     rel_g = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+
+    if args.calib_copy:
+        # known traffic: 256 MiB read + 256 MiB written per launch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # dominant single-kernel stage and its algorithmic bytes per launch
     nf = info32["nf"]
@@ -133,7 +145,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom, info32),
         "avg_launch_ms": round(dom_ms, 4),
         "alg_bytes_per_launch": alg_bytes[dom],
     }
@@ -173,6 +185,7 @@ def main():
             "nf": nf,
             "w": info32["w"],
             "tile": info32["tile"],
+            "spreader": "%s, %d slices/bin (autotuned)" % (info32["spreader"], info32["spread_parts"]),
         },
         "points_per_s": round(ev32 * N, 1),
         "fp64_plain": {
@@ -192,6 +205,23 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(stage, info):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)
+    for the spreader configuration in use, or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "nuwu3d_n2p20_stealthy24_hbm_traffic.md")
+    cfg = "%s:%d" % (info.get("spreader"), info.get("spread_parts"))
+    key = {"spread": "spread_traffic_bytes_per_launch[%s]:" % cfg, "interp": "interp_traffic_bytes_per_launch:"}[stage]
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith(key):
+                    return int(float(line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(radii, kp, Sk, V, L, p, N):

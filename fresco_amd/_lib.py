@@ -67,6 +67,7 @@ SIGNATURES = {
     "fresco_pot_get_rho": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_set_targets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fresco_pot_spread_config": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     "fresco_pot_ndof": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_nmodes": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_plan_info": (
@@ -107,7 +108,7 @@ SIGNATURES = {
     ),
 }
 
-N_STAGES = 7
+N_STAGES = 8
 
 OPT_TOL, OPT_MAXSTEP, OPT_MAX_ITER, OPT_IPRINT, OPT_VERBOSITY, OPT_H0, OPT_MAX_F_RISE, OPT_USE_RELATIVE_F = range(8)
 

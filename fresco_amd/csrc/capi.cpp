@@ -336,6 +336,14 @@ int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, v
   });
 }
 
+int fresco_pot_spread_config(fresco_pot* pot, int* mode, int* parts) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!mode || !parts) throw fresco_hip::InvalidArgument("null pointer");
+    p->spread_config(mode, parts);
+  });
+}
+
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof) {
   return guarded([&]() { *ndof = checked(pot)->ndof(); });
 }

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "nufft.hpp"
+#include "prof.hpp"
 
 namespace fresco_hip {
 
@@ -681,18 +682,68 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     }
     halo_tab.allocate(tabs.size());
     halo_tab.upload(tabs.data(), tabs.size());
-    // about three resident waves' worth of (bin, slice) work items
-    const int per_cu = std::max(1, (int)(160 * 1024 / (HT_ELEMS * sizeof(T))));
-    const int target = 3 * 256 * per_cu;
-    halo_parts = std::max(1, std::min(16, (target + g.nbins - 1) / g.nbins));
-    halo_buf.allocate((size_t)g.nbins * halo_parts * HT_ELEMS);
-    // measured on MI355X (N = 2^20, w = 8): the halo spreader wins on grids
-    // of few bins (637 bins: 0.38 vs 0.68 ms), the register tiles on large
-    // grids (9826 bins: 0.52 vs 0.72 ms, the merge traffic grows with the
-    // grid); FRESCO_SPREAD3=reg|halo overrides for measurement
-    mode = g.nbins <= 4096 ? HALO : REG;
-    if (const char* env = std::getenv("FRESCO_SPREAD3")) mode = std::string(env) == "reg" ? REG : HALO;
+    // slices per bin: more slices, more wavefronts in flight but more halo
+    // tiles to write and merge; the autotuner picks among these
+    max_halo_parts = std::max(1, std::min(16, (16384 + g.nbins - 1) / g.nbins));
+    halo_buf.allocate((size_t)g.nbins * max_halo_parts * HT_ELEMS);
+    mode = HALO;
+    halo_parts = std::min(4, max_halo_parts);
+    tuned = false;
+    // FRESCO_SPREAD3=reg | halo | halo:<slices> pins the choice (reproducible
+    // rounding across processes) instead of timing the candidates
+    if (const char* env = std::getenv("FRESCO_SPREAD3")) {
+      const std::string v(env);
+      if (v == "reg") {
+        mode = REG;
+      } else if (v.rfind("halo", 0) == 0) {
+        mode = HALO;
+        if (v.size() > 5) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 5)));
+      }
+      tuned = true;
+    }
   }
+}
+
+// Time every spreader configuration once on the caller's sorted points (a
+// warm-up launch, then one launch between HIP events) and keep the fastest.
+// Runs on the first spread of a plan whose grid admits both 3D spreaders.
+template <typename T>
+void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
+  struct Cand {
+    int mode, parts;
+  };
+  std::vector<Cand> cands;
+  cands.push_back({REG, 1});
+  for (int sl : {1, 2, 3, 4, 6, 8, 12, 16})
+    if (sl <= max_halo_parts) cands.push_back({HALO, sl});
+  hipEvent_t e0, e1;
+  FRESCO_HIP_CHECK(hipEventCreate(&e0));
+  FRESCO_HIP_CHECK(hipEventCreate(&e1));
+  float best = 1e30f;
+  Cand pick = cands[0];
+  tuned = true;  // the launches below dispatch on mode / halo_parts directly
+  const bool prof = prof_enabled();
+  prof_set(false);
+  for (const Cand& c : cands) {
+    mode = c.mode;
+    halo_parts = c.parts;
+    spread(p, grid, s);
+    FRESCO_HIP_CHECK(hipEventRecord(e0, s));
+    spread(p, grid, s);
+    FRESCO_HIP_CHECK(hipEventRecord(e1, s));
+    FRESCO_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FRESCO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  prof_set(prof);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  mode = pick.mode;
+  halo_parts = pick.parts;
 }
 
 template <typename T, int D, int WMAX>
@@ -712,6 +763,7 @@ static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* gr
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
+  ProfScope ps(ST_SPREAD, s);
   hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins), dim3(64), sp.lds_bytes, s, p.scell.get(), p.rec.get(),
                      p.bin_start.get(), sp.cand.get(), a, grid);
   FRESCO_HIP_CHECK(hipGetLastError());
@@ -764,6 +816,7 @@ template <typename T>
 void Spreader<T>::spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
   const int d = geom.d;
   const bool small = es.w <= 8;
+  if (d == 3 && halo_ok && !tuned) autotune(p, grid, s);
   if (d == 3) {
     if (launch_spread3_halo<T>(*this, p, grid, s)) return;
     if (launch_spread3_reg<T>(*this, p, grid, s)) return;
@@ -1334,9 +1387,13 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
-  hipLaunchKernelGGL(k_spread3_halo<T>, dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
-                     p.bin_start.get(), a, sp.halo_buf.get());
-  FRESCO_HIP_CHECK(hipGetLastError());
+  {
+    ProfScope ps(ST_SPREAD, s);
+    hipLaunchKernelGGL(k_spread3_halo<T>, dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
+                       p.bin_start.get(), a, sp.halo_buf.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+  ProfScope pm(ST_MERGE, s);
   const size_t n = (size_t)g.ncells();
   hipLaunchKernelGGL(k_merge_halo<T>, dim3((unsigned)std::min<size_t>(blocks_for(n, 256), 16384)), dim3(256), 0, s,
                      sp.halo_buf.get(), sp.halo_tab.get(), sp.halo_parts, a, grid);
@@ -1363,10 +1420,14 @@ static void launch_spread3_reg_w(const Spreader<T>& sp, const SortedPoints<T>& p
   a.k.two_over_w = (T)(2.0 / sp.es.w);
   const int parts = sp.parts;
   T* out = parts > 1 ? sp.part_buf.get() : grid;
-  hipLaunchKernelGGL((k_spread3_reg<T, WMAX>), dim3(g.nbins, parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
-                     p.bin_start.get(), sp.cand.get(), a, out);
-  FRESCO_HIP_CHECK(hipGetLastError());
+  {
+    ProfScope ps(ST_SPREAD, s);
+    hipLaunchKernelGGL((k_spread3_reg<T, WMAX>), dim3(g.nbins, parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
+                       p.bin_start.get(), sp.cand.get(), a, out);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
   if (parts > 1) {
+    ProfScope pm(ST_MERGE, s);
     const size_t nc4 = (size_t)g.ncells();
     hipLaunchKernelGGL(k_sum_parts<T>, dim3((unsigned)std::min<size_t>(blocks_for(nc4, 256), 16384)), dim3(256), 0,
                        s, out, parts, (int64_t)nc4, grid);

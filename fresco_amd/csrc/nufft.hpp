@@ -100,6 +100,8 @@ class Spreader {
   // complex strengths str[n] in original point order (times the record
   // weight) onto a complex grid[ncells], fully overwritten.
   void spread_complex(const SortedPoints<T>& p, const Cplx<T>* str, Cplx<T>* grid, hipStream_t s) const;
+  // time the 3D spreader configurations on these points, keep the fastest
+  void autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
 
   GridGeom geom;
   EsParams es;
@@ -111,9 +113,11 @@ class Spreader {
   DeviceBuffer<T> part_buf;  // [parts][ncells] partial grids when parts > 1
   // input-driven halo spreader (3D, w <= 8)
   enum Mode { REG = 0, HALO = 1 };
-  int mode = REG;
+  mutable int mode = REG;
   bool halo_ok = false;
-  int halo_parts = 1;          // point-range slices per bin
+  mutable bool tuned = true;   // false: time the candidates on the first spread
+  mutable int halo_parts = 1;  // point-range slices per bin
+  int max_halo_parts = 1;
   DeviceBuffer<T> halo_buf;    // [nbins][halo_parts][halo tile]
   DeviceBuffer<int> halo_tab;  // per-dimension coverage tables for the merge
 };

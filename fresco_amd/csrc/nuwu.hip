@@ -246,10 +246,7 @@ class NUwU final : public Potential {
       ProfScope p(ST_SORT, s);
       pts_.set(x, map_e_, c_.get(), s);
     }
-    {
-      ProfScope p(ST_SPREAD, s);
-      spreader_.spread(pts_, grid_.get(), s);
-    }
+    spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
     {
       ProfScope p(ST_FFT_FWD, s);
       r2c_.exec(grid_.get(), S_.get(), s);
@@ -268,10 +265,7 @@ class NUwU final : public Potential {
       ProfScope p(ST_SORT, s);
       pts_.set(x, map_g_, c_.get(), s);
     }
-    {
-      ProfScope p(ST_SPREAD, s);
-      spreader_.spread(pts_, grid_.get(), s);
-    }
+    spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
     {
       ProfScope p(ST_FFT_FWD, s);
       r2c_.exec(grid_.get(), S_.get(), s);
@@ -342,6 +336,11 @@ class NUwU final : public Potential {
     }
     *w = es_.w;
     *beta = es_.beta;
+  }
+
+  void spread_config(int* mode, int* parts) const override {
+    *mode = spreader_.mode;
+    *parts = spreader_.mode == Spreader<T>::HALO ? spreader_.halo_parts : spreader_.parts;
   }
 
   void target_grads(double* dS0, double* dV, hipStream_t s) override {

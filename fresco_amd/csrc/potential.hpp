@@ -43,6 +43,12 @@ class Potential {
     (void)s;
     throw RuntimeFailure("this potential has no S0 / V targets");
   }
+  // spreader configuration in use: mode 0 register tiles / owner tiles,
+  // 1 halo tiles; parts = point-range slices per bin (-1 when not applicable)
+  virtual void spread_config(int* mode, int* parts) const {
+    *mode = -1;
+    *parts = -1;
+  }
   virtual void plan_info(int64_t* nf, int* w, double* beta, int* tile) const {
     for (int a = 0; a < 3; ++a) {
       nf[a] = 0;

@@ -34,7 +34,7 @@ hipEvent_t get_event() {
 }  // namespace
 
 const char* stage_name(int st) {
-  static const char* names[ST_COUNT] = {"bin_sort", "spread", "fft_fwd", "zero_fill", "loss", "fft_inv", "interp"};
+  static const char* names[ST_COUNT] = {"bin_sort", "spread", "fft_fwd", "zero_fill", "loss", "fft_inv", "interp", "spread_merge"};
   return (st >= 0 && st < ST_COUNT) ? names[st] : "";
 }
 

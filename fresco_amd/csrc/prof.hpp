@@ -6,7 +6,8 @@
 
 namespace fresco_hip {
 
-enum Stage { ST_SORT = 0, ST_SPREAD, ST_FFT_FWD, ST_ZERO, ST_LOSS, ST_FFT_INV, ST_INTERP, ST_COUNT };
+// ST_MERGE: the second kernel of a two-kernel spread (partial-tile sums)
+enum Stage { ST_SORT = 0, ST_SPREAD, ST_FFT_FWD, ST_ZERO, ST_LOSS, ST_FFT_INV, ST_INTERP, ST_MERGE, ST_COUNT };
 
 const char* stage_name(int st);
 bool prof_enabled();

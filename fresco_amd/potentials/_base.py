@@ -187,7 +187,11 @@ class BasePotential:
         w = ctypes.c_int()
         beta = ctypes.c_double()
         _lib.check(lib.fresco_pot_plan_info(self._h, nf, ctypes.byref(w), ctypes.byref(beta), tile))
-        return {"nf": list(nf), "w": int(w.value), "beta": float(beta.value), "tile": list(tile)}
+        mode = ctypes.c_int()
+        parts = ctypes.c_int()
+        _lib.check(lib.fresco_pot_spread_config(self._h, ctypes.byref(mode), ctypes.byref(parts)))
+        return {"nf": list(nf), "w": int(w.value), "beta": float(beta.value), "tile": list(tile),
+                "spreader": {0: "tiles", 1: "halo"}.get(mode.value), "spread_parts": int(parts.value)}
 
     def __del__(self):
         h = getattr(self, "_h", None)

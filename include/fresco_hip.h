@@ -198,6 +198,14 @@ int fresco_pot_backward(fresco_pot* pot, double* dS0, double* dV, void* stream);
  */
 int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, void* stream);
 
+/*
+ * Spreader configuration of a NUFFT-based potential after its autotuning
+ * (done on the first evaluation): mode 0 = register / owner tiles, 1 = halo
+ * tiles with `parts` point-range slices per bin; -1 when not applicable.
+ * No reference counterpart (FINUFFT chooses its spreading internally).
+ */
+int fresco_pot_spread_config(fresco_pot* pot, int* mode, int* parts);
+
 /* Number of degrees of freedom the potential expects (n*d for points, prod(N) for grids). */
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof);
 
@@ -279,7 +287,7 @@ int fresco_opt_destroy(fresco_opt* opt);
 /*
  * Stage profiler (HIP events recorded on the evaluation's stream around each
  * stage: 0 bin_sort, 1 spread, 2 fft_fwd, 3 zero_fill, 4 loss, 5 fft_inv,
- * 6 interp).  fresco_prof_get returns the stage name, the summed duration in
+ * 6 interp, 7 spread_merge = the partial-tile sum kernel of two-kernel spreads).  fresco_prof_get returns the stage name, the summed duration in
  * milliseconds and the number of timed launches since the last reset.
  * No reference counterpart (the reference has no timers; its only progress
  * output is the optimizer's iprint line, lbfgs.cpp:56-62).
