@@ -57,7 +57,7 @@ SIGNATURES = {
     "fresco_nuwnu_create": (
         ctypes.c_int,
         [c_double_p, ctypes.c_int64, ctypes.c_int, c_double_p, ctypes.c_int64, c_double_p, c_double_p,
-         ctypes.c_int64, c_double_p, ctypes.c_int, c_void_pp],
+         ctypes.c_int64, c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, c_void_pp],
     ),
     "fresco_pot_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p, ctypes.c_void_p]),
     "fresco_pot_energy_gradient": (

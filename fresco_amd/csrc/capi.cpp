@@ -129,14 +129,16 @@ int fresco_uwnu_create(const int64_t* shape, int d, const double* K, int64_t nkd
 }
 
 int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, int64_t nkd, const double* Sk,
-                        const double* V, int64_t m, const double* L, int precision, fresco_pot** out) {
+                        const double* V, int64_t m, const double* L, double eps, int precision, int method,
+                        double sigma, fresco_pot** out) {
   return guarded([&]() {
     if (!out) throw fresco_hip::InvalidArgument("out is null");
     *out = nullptr;
     if (!radii || !K || !Sk || !V || !L) throw fresco_hip::InvalidArgument("null array argument");
     if (precision < 0 || precision > 2) throw fresco_hip::InvalidArgument("unknown precision");
+    if (method == 0 && !(eps > 0)) throw fresco_hip::InvalidArgument("eps must be positive");
     auto h = std::make_unique<fresco_pot>();
-    h->impl = fresco_hip::make_nuwnu(radii, n, d, K, nkd, Sk, V, m, L, precision);
+    h->impl = fresco_hip::make_nuwnu(radii, n, d, K, nkd, Sk, V, m, L, eps, precision, method, sigma);
     *out = h.release();
   });
 }

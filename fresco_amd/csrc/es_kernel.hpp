@@ -35,6 +35,10 @@ EsParams es_params(double eps, double sigma, int extra_w = 0);
 // Psi(k) for k = 0..kmax on a periodic grid of n cells (float64).
 std::vector<double> es_deconv(const EsParams& p, int n, int kmax);
 
+// Continuous transform of the kernel on [-1, 1]: int phi(z) cos(u z) dz for
+// each u (float64, Gauss-Legendre), the deconvolution of type-3 transforms.
+std::vector<double> es_ft(const EsParams& p, const std::vector<double>& u);
+
 // Smallest even 2,3,5-smooth integer >= max(target, 2w).
 int next_smooth_even(int target, int w);
 

@@ -92,6 +92,21 @@ std::vector<double> es_deconv(const EsParams& p, int n, int kmax) {
   return out;
 }
 
+std::vector<double> es_ft(const EsParams& p, const std::vector<double>& u) {
+  const int q = 4 * p.w + 64;
+  std::vector<double> z, wt;
+  gauss_legendre(q, z, wt);
+  std::vector<double> ph(q);
+  for (int i = 0; i < q; ++i) ph[i] = wt[i] * std::exp(p.beta * (std::sqrt(1.0 - z[i] * z[i]) - 1.0));
+  std::vector<double> out(u.size());
+  for (size_t k = 0; k < u.size(); ++k) {
+    double s = 0.0;
+    for (int i = 0; i < q; ++i) s += ph[i] * std::cos(u[k] * z[i]);
+    out[k] = s;
+  }
+  return out;
+}
+
 int next_smooth_even(int target, int w) {
   int n = std::max(target, 2 * w);
   if (n % 2) ++n;

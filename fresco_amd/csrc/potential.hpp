@@ -104,8 +104,10 @@ std::unique_ptr<Potential> make_uwu(const int64_t* shape, int d, const double* S
 std::unique_ptr<Potential> make_uwnu(const int64_t* shape, int d, const double* K, int64_t nkd, const double* Sk,
                                      const double* V, int64_t nk, const double* L, double eps, int precision,
                                      double sigma);
+// method 0: type-3 NUFFT (eps, sigma); 1: exact sums with float64 phases
 std::unique_ptr<Potential> make_nuwnu(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
-                                      const double* Sk, const double* V, int64_t m, const double* L, int precision);
+                                      const double* Sk, const double* V, int64_t m, const double* L, double eps,
+                                      int precision, int method, double sigma);
 
 // kind 0 inverse power (p1 = exponent a), 1 WCA (p1 = sigma); method 1
 // cartesian, 2 periodic (box[d]); cells != 0: cell lists (periodic only)

@@ -523,11 +523,11 @@ __global__ void __launch_bounds__(TB)
 }
 
 template <typename T>
-class NUwNU final : public Potential {
+class NUwNUDirect final : public Potential {
  public:
   // the k vectors set the mode count, Nk = len(K) / d (nuwnu.hpp:40)
-  NUwNU(const double* radii, int64_t n, int d, const double* K, int64_t nkd, const double* Sk, const double* V,
-        int64_t m, const double* L)
+  NUwNUDirect(const double* radii, int64_t n, int d, const double* K, int64_t nkd, const double* Sk, const double* V,
+              int64_t m, const double* L)
       : N_(n), d_(d), nk_(d >= 1 && d <= 3 ? nkd / d : 0) {
     if (d < 1 || d > 3) throw InvalidArgument("Invalid dimension: " + std::to_string(d));
     if (n <= 0 || nk_ <= 0) throw InvalidArgument("need points and k vectors");
@@ -618,6 +618,347 @@ class NUwNU final : public Potential {
   HostScalar host_;
 };
 
+
+// ---------------------------------------------------------------------------
+// NUwNU by a type-3 NUFFT (nuwnu.hpp:77-78: FINUFFT type 3, +1 for rho, -1
+// for the gradient).  With x = C + xt (C = 2 pi: the +L-shifted coordinates
+// lie in [pi, 3 pi]) and k = D + kt (D the centre of the k box):
+//   rho(k) = e^{i k.C} sum_j [c_j e^{i D.xt_j}] e^{i kt.xt_j}
+// The bracket is spread with the ES kernel onto an outer periodic grid of
+// spacing h (xt_j at cell xt_j / h + nf1/2), whose Fourier series at
+// s = kt h is an inner type-2 NUFFT (the outer grid's cells are its modes);
+// deconvolving by the kernel's continuous transform gives the sum:
+//   sum_j c'_j e^{i kt.xt_j} = prod_a 2 / (w phihat(kt_a h_a w / 2)) B(kt h)
+// Sizes per dimension (X = pi, S = half width of the k box):
+//   nf1 >= 2 sigma X S / pi + w + 1,  h = 2 X / (nf1 - w),  nf2 = sigma nf1.
+// The gradient grad_{j,a} = Re(sum_q k_{q,a} f_q e^{-i k_q.x_j}) runs the
+// chain backwards for each component a: conj(deconvolution) -> inner type 1
+// (spread at s, forward FFT, deconvolve) -> interpolation at xt_j ->
+// times e^{-i D.xt_j}.
+// ---------------------------------------------------------------------------
+
+struct T3Map {
+  int d;
+  double L[3];
+  double D[3];
+};
+
+// c'_j = c_j e^{i D.xt_j}, xt = (p - round(p/L)) 2 pi / L
+template <typename T>
+__global__ void k_t3_strength(const double* __restrict__ p, const double* __restrict__ c, int64_t n, T3Map m,
+                              Cplx<T>* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double ph = 0.0;
+    for (int a = 0; a < m.d; ++a) {
+      const double v = p[j * m.d + a];
+      ph += m.D[a] * (v - round(v / m.L[a])) * (2.0 * M_PI / m.L[a]);
+    }
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    out[j] = Cplx<T>{(T)(c[j] * cs), (T)(c[j] * sn)};
+  }
+}
+
+// complex modes b (outer grid, index l, m = l - n/2, first dimension fastest)
+// -> inner fine grid at m mod nf, times 1/Psi(m)
+template <typename T>
+__global__ void k_t3_pad(const Cplx<T>* __restrict__ b, ModeGridDev g, int64_t cnt, Cplx<T>* __restrict__ fine) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = j, fi = 0, mul = 1;
+    double wgt = 1.0;
+    for (int a = 0; a < g.d; ++a) {
+      const int64_t i = r % g.n[a];
+      r /= g.n[a];
+      const int64_t mm = i - g.n[a] / 2;
+      wgt *= g.ipsi[a][mm < 0 ? -mm : mm];
+      fi += mul * (mm < 0 ? mm + g.nf[a] : mm);
+      mul *= g.nf[a];
+    }
+    const Cplx<T> v = b[j];
+    fine[fi] = Cplx<T>{(T)(wgt * (double)v.re), (T)(wgt * (double)v.im)};
+  }
+}
+
+// inner fine grid -> outer grid b_l = fine[m mod nf] / Psi(m)
+template <typename T>
+__global__ void k_t3_extract(const Cplx<T>* __restrict__ fine, ModeGridDev g, int64_t cnt, Cplx<T>* __restrict__ b) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = j, fi = 0, mul = 1;
+    double wgt = 1.0;
+    for (int a = 0; a < g.d; ++a) {
+      const int64_t i = r % g.n[a];
+      r /= g.n[a];
+      const int64_t mm = i - g.n[a] / 2;
+      wgt *= g.ipsi[a][mm < 0 ? -mm : mm];
+      fi += mul * (mm < 0 ? mm + g.nf[a] : mm);
+      mul *= g.nf[a];
+    }
+    const Cplx<T> v = fine[fi];
+    b[j] = Cplx<T>{(T)(wgt * (double)v.re), (T)(wgt * (double)v.im)};
+  }
+}
+
+// rho_q = corr_q B_q (complex, float64 interleaved)
+__global__ void k_t3_post(const double* __restrict__ B, const double* __restrict__ corr, int64_t nk,
+                          double* __restrict__ rho) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nk; q += (int64_t)gridDim.x * blockDim.x) {
+    const double br = B[2 * q], bi = B[2 * q + 1], cr = corr[2 * q], ci = corr[2 * q + 1];
+    rho[2 * q] = br * cr - bi * ci;
+    rho[2 * q + 1] = br * ci + bi * cr;
+  }
+}
+
+// adjoint strengths u_q = k_{q,a} f_q conj(corr_q)
+template <typename T>
+__global__ void k_t3_adj_strength(const Cplx<T>* __restrict__ f, const double* __restrict__ corr,
+                                  const double* __restrict__ kv, int d, int a, int64_t nk, Cplx<T>* __restrict__ u) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nk; q += (int64_t)gridDim.x * blockDim.x) {
+    const double fr = (double)f[q].re, fi = (double)f[q].im, cr = corr[2 * q], ci = -corr[2 * q + 1];
+    const double ka = kv[q * d + a];
+    u[q] = Cplx<T>{(T)(ka * (fr * cr - fi * ci)), (T)(ka * (fr * ci + fi * cr))};
+  }
+}
+
+// grad[j d + a] = Re(e^{-i D.xt_j} H_j)
+__global__ void k_t3_grad(const double* __restrict__ p, const double* __restrict__ H, int64_t n, T3Map m, int a,
+                          double* __restrict__ grad) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double ph = 0.0;
+    for (int b = 0; b < m.d; ++b) {
+      const double v = p[j * m.d + b];
+      ph += m.D[b] * (v - round(v / m.L[b])) * (2.0 * M_PI / m.L[b]);
+    }
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    // Re((cs - i sn)(hr + i hi)) = cs hr + sn hi
+    grad[j * m.d + a] = cs * H[2 * j] + sn * H[2 * j + 1];
+  }
+}
+
+template <typename T>
+class NUwNUType3 final : public Potential {
+ public:
+  NUwNUType3(const double* radii, int64_t n, int d, const double* K, int64_t nkd, const double* Sk, const double* V,
+             int64_t m, const double* L, double eps, double sigma)
+      : N_(n), d_(d), nk_(d >= 1 && d <= 3 ? nkd / d : 0) {
+    if (d < 1 || d > 3) throw InvalidArgument("Invalid dimension: " + std::to_string(d));
+    if (n <= 0 || nk_ <= 0) throw InvalidArgument("need points and k vectors");
+    if (m < nk_) throw InvalidArgument("Sk / V hold fewer than len(K) / d entries");
+    if (sigma <= 0) sigma = 2.0;
+    es_ = es_params(eps, sigma);
+    const int w = es_.w;
+    tm_.d = d;
+    for (int a = 0; a < 3; ++a) {
+      tm_.L[a] = 1.0;
+      tm_.D[a] = 0.0;
+    }
+    // k box per dimension
+    double S[3] = {0, 0, 0};
+    for (int a = 0; a < d; ++a) {
+      if (!(L[a] > 0)) throw InvalidArgument("box lengths must be positive");
+      tm_.L[a] = L[a];
+      double lo = K[a], hi = K[a];
+      for (int64_t q = 0; q < nk_; ++q) {
+        lo = std::min(lo, K[q * d + a]);
+        hi = std::max(hi, K[q * d + a]);
+      }
+      tm_.D[a] = 0.5 * (lo + hi);
+      S[a] = 0.5 * (hi - lo);
+    }
+    // outer and inner grids
+    const double X = M_PI;
+    int nf1[3] = {1, 1, 1}, nf2[3] = {1, 1, 1};
+    for (int a = 0; a < d; ++a) {
+      nf1[a] = next_smooth_even((int)std::ceil(2.0 * sigma * X * S[a] / M_PI) + w + 1, w);
+      h_[a] = 2.0 * X / (nf1[a] - w);
+      nf2[a] = next_smooth_even((int)std::ceil(sigma * nf1[a]), w);
+    }
+    g1_ = make_geom(d, nf1, w, (int)sizeof(T));
+    g2_ = make_geom(d, nf2, w, (int)sizeof(T));
+    src_.init(g1_, es_, n);
+    sp1_.init(g1_, es_);
+    tgt_.init(g2_, es_, nk_);
+    sp2_.init(g2_, es_);
+    // sources: cell xt/h + nf1/2 on the outer grid
+    for (int a = 0; a < 3; ++a) {
+      const double La = a < d ? L[a] : 1.0;
+      const double ha = a < d ? h_[a] : 1.0;
+      const double n1 = a < d ? nf1[a] : 1.0;
+      cmap_.L[a] = La;
+      cmap_.shift[a] = La * ha * n1 / (4.0 * M_PI);
+      cmap_.scale[a] = 4.0 * M_PI * M_PI / (La * ha * n1);
+    }
+    cmap_.wrap = 1;
+    // targets s = (k - D) h on the inner grid, fixed: sorted once
+    std::vector<double> s((size_t)nk_ * d), u;
+    std::vector<double> corr(2 * (size_t)nk_);
+    std::vector<double> fac((size_t)nk_, 1.0);
+    for (int a = 0; a < d; ++a) {
+      u.resize(nk_);
+      for (int64_t q = 0; q < nk_; ++q) {
+        const double kt = K[q * d + a] - tm_.D[a];
+        s[(size_t)q * d + a] = kt * h_[a];
+        u[q] = kt * h_[a] * w / 2.0;
+      }
+      const std::vector<double> ph = es_ft(es_, u);
+      for (int64_t q = 0; q < nk_; ++q) fac[q] *= 2.0 / (w * ph[q]);
+    }
+    for (int64_t q = 0; q < nk_; ++q) {
+      double ang = 0.0;
+      for (int a = 0; a < d; ++a) ang += K[q * d + a] * 2.0 * M_PI;  // e^{i k.C}, C = 2 pi
+      corr[2 * q] = fac[q] * std::cos(ang);
+      corr[2 * q + 1] = fac[q] * std::sin(ang);
+    }
+    DeviceBuffer<double> sd(s.size());
+    sd.upload(s.data(), s.size());
+    CoordMap tmap;
+    tmap.wrap = 0;
+    tgt_.set(sd.get(), tmap, nullptr, nullptr);
+    FRESCO_HIP_CHECK(hipDeviceSynchronize());
+    corr_.allocate(corr.size());
+    corr_.upload(corr.data(), corr.size());
+    k_.allocate((size_t)nk_ * d);
+    k_.upload(K, (size_t)nk_ * d);
+    // inner deconvolution over the outer grid's modes
+    int64_t M1 = 1;
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> ps = a < d ? es_deconv(es_, nf2[a], nf1[a] / 2 + 1) : std::vector<double>(1, 1.0);
+      for (auto& v : ps) v = 1.0 / v;
+      ipsi_[a].allocate(ps.size());
+      ipsi_[a].upload(ps.data(), ps.size());
+      mg_.ipsi[a] = ipsi_[a].get();
+      mg_.n[a] = a < d ? nf1[a] : 1;
+      mg_.nf[a] = a < d ? nf2[a] : 1;
+      M1 *= mg_.n[a];
+    }
+    mg_.d = d;
+    M1_ = M1;
+    const bool dbl = sizeof(T) == 8;
+    fwd_.create(FftPlan::C2C_FWD, dbl, d, g2_.nf);
+    inv_.create(FftPlan::C2C_INV, dbl, d, g2_.nf);
+    outer_.allocate(g1_.ncells());
+    fine_.allocate(g2_.ncells());
+    fine2_.allocate(g2_.ncells());
+    str_.allocate(n);
+    fk_.allocate(nk_);
+    uk_.allocate(nk_);
+    B_.allocate(2 * (size_t)nk_);
+    H_.allocate(2 * (size_t)n);
+    // weights (nuwnu.hpp:81-105)
+    std::vector<double> c(n, 1.0);
+    double sum = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      for (int a = 0; a < d; ++a) {
+        if (d == 2) c[i] = c[i] * M_PI * radii[i] * radii[i];
+        else if (d == 3) c[i] = c[i] * M_PI * radii[i] * radii[i] * radii[i] * 4.0 / 3.0;
+      }
+      sum += c[i];
+    }
+    for (auto& v : c) v *= (double)n / sum;
+    c_.allocate(n);
+    c_.upload(c.data(), n);
+    Sk_.allocate(nk_);
+    V_.allocate(nk_);
+    Sk_.upload(Sk, nk_);
+    V_.upload(V, nk_);
+    rho_.allocate(2 * (size_t)nk_);
+    nb_ = rblocks(nk_);
+    epart_.allocate(nb_);
+    e_.allocate(1);
+  }
+  int64_t ndof() const override { return N_ * d_; }
+  int64_t nmodes() const override { return nk_; }
+
+  double energy(const double* x, hipStream_t s) override {
+    forward(x, s);
+    hipLaunchKernelGGL((k_pts_loss<T, 0>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), nk_,
+                       (double)N_, 0.0, 0.0, fk_.get(), epart_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+
+  double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    forward(x, s);
+    // f = (-4i/N) V D rho / S0 (nuwnu.hpp:228,247)
+    hipLaunchKernelGGL((k_pts_loss<T, 1>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), nk_,
+                       (double)N_, 0.0, -4.0 / (double)N_, fk_.get(), epart_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    for (int a = 0; a < d_; ++a) {
+      hipLaunchKernelGGL(k_t3_adj_strength<T>, dim3(rblocks(nk_)), dim3(TB), 0, s, fk_.get(), corr_.get(), k_.get(),
+                         d_, a, nk_, uk_.get());
+      FRESCO_HIP_CHECK(hipGetLastError());
+      // inner type 1 (-1) at s, then the outer grid's modes
+      sp2_.spread_complex(tgt_, uk_.get(), fine_.get(), s);
+      fwd_.exec(fine_.get(), fine2_.get(), s);
+      hipLaunchKernelGGL(k_t3_extract<T>, dim3(rblocks(M1_)), dim3(TB), 0, s, fine2_.get(), mg_, M1_, outer_.get());
+      FRESCO_HIP_CHECK(hipGetLastError());
+      interpolate_complex(src_, outer_.get(), H_.get(), s);
+      hipLaunchKernelGGL(k_t3_grad, dim3(rblocks(N_)), dim3(TB), 0, s, x, H_.get(), N_, tm_, a, grad);
+      FRESCO_HIP_CHECK(hipGetLastError());
+    }
+    return host_.fetch(e_.get(), s);
+  }
+
+  void get_rho(double* rho, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(rho, rho_.get(), 2 * nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  void target_grads(double* dS0, double* dV, hipStream_t s) override {
+    launch_target_grads(rho_.get(), Sk_.get(), V_.get(), nk_, (double)N_, nullptr, dS0, dV, s);
+  }
+
+  void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  void plan_info(int64_t* nf, int* w, double* beta, int* tile) const override {
+    // the inner (upsampled) grid; the outer spreading grid is nf / sigma
+    for (int a = 0; a < 3; ++a) {
+      nf[a] = g2_.nf[a];
+      tile[a] = g2_.tile[a];
+    }
+    *w = es_.w;
+    *beta = es_.beta;
+  }
+
+ private:
+  void forward(const double* x, hipStream_t s) {
+    src_.set(x, cmap_, nullptr, s);
+    hipLaunchKernelGGL(k_t3_strength<T>, dim3(rblocks(N_)), dim3(TB), 0, s, x, c_.get(), N_, tm_, str_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    sp1_.spread_complex(src_, str_.get(), outer_.get(), s);
+    fill_zero(fine_.get(), g2_.ncells(), s);
+    hipLaunchKernelGGL(k_t3_pad<T>, dim3(rblocks(M1_)), dim3(TB), 0, s, outer_.get(), mg_, M1_, fine_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    inv_.exec(fine_.get(), fine2_.get(), s);
+    interpolate_complex(tgt_, fine2_.get(), B_.get(), s);
+    hipLaunchKernelGGL(k_t3_post, dim3(rblocks(nk_)), dim3(TB), 0, s, B_.get(), corr_.get(), nk_, rho_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+
+  int64_t N_;
+  int d_;
+  int64_t nk_;
+  EsParams es_;
+  T3Map tm_{};
+  double h_[3] = {1, 1, 1};
+  GridGeom g1_, g2_;
+  SortedPoints<T> src_, tgt_;
+  Spreader<T> sp1_, sp2_;
+  CoordMap cmap_;
+  ModeGridDev mg_{};
+  int64_t M1_ = 1;
+  DeviceBuffer<double> ipsi_[3];
+  FftPlan fwd_, inv_;
+  DeviceBuffer<Cplx<T>> outer_, fine_, fine2_, str_, fk_, uk_;
+  DeviceBuffer<double> corr_, k_, B_, H_, c_, Sk_, V_, rho_, epart_, e_;
+  int nb_ = 1;
+  HostScalar host_;
+};
+
 }  // namespace
 
 std::unique_ptr<Potential> make_uwu(const int64_t* shape, int d, const double* Sk, const double* V, int64_t m,
@@ -634,9 +975,15 @@ std::unique_ptr<Potential> make_uwnu(const int64_t* shape, int d, const double* 
 }
 
 std::unique_ptr<Potential> make_nuwnu(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
-                                      const double* Sk, const double* V, int64_t m, const double* L, int precision) {
-  if (precision == 1) return std::make_unique<NUwNU<float>>(radii, n, d, K, nkd, Sk, V, m, L);
-  return std::make_unique<NUwNU<double>>(radii, n, d, K, nkd, Sk, V, m, L);
+                                      const double* Sk, const double* V, int64_t m, const double* L, double eps,
+                                      int precision, int method, double sigma) {
+  if (method == 1) {
+    if (precision == 1) return std::make_unique<NUwNUDirect<float>>(radii, n, d, K, nkd, Sk, V, m, L);
+    return std::make_unique<NUwNUDirect<double>>(radii, n, d, K, nkd, Sk, V, m, L);
+  }
+  if (method != 0) throw InvalidArgument("unknown NUwNU method " + std::to_string(method));
+  if (precision == 1) return std::make_unique<NUwNUType3<float>>(radii, n, d, K, nkd, Sk, V, m, L, eps, sigma);
+  return std::make_unique<NUwNUType3<double>>(radii, n, d, K, nkd, Sk, V, m, L, eps, sigma);
 }
 
 }  // namespace fresco_hip

@@ -21,12 +21,13 @@ class NUwNU(BasePotential):
     Re(sum_q K_q,a (-4i/N) V D rho / S0 exp(-i K_q . x_j)) with no weight
     factor (nuwnu.hpp:184-290).  Nk = len(K) / d.
 
-    ``eps`` is kept for the constructor signature; the sums are exact.
-    Keyword-only ``precision``: "fp64" (default) or "fp32" (FP32 sin/cos
-    accumulated in blocks of 256 terms, float64 across blocks).
+    As in the reference, rho and the gradient come from type-3 NUFFTs at
+    tolerance ``eps``.  Keyword-only extensions: ``precision`` ("fp64"
+    (default) or "fp32" grids), ``method`` ("nufft" (default) or "direct":
+    exact sums with float64 phases, O(N Nk)) and ``sigma`` (upsampling).
     """
 
-    def __init__(self, radii, K, Sk, V, L, eps=1e-6, *, precision="fp64"):
+    def __init__(self, radii, K, Sk, V, L, eps=1e-6, *, precision="fp64", method="nufft", sigma=None):
         self.radii = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).ravel())
         self.N = len(self.radii)
         self.K = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel())
@@ -37,13 +38,19 @@ class NUwNU(BasePotential):
         self.ndim = len(self.L)
         self.eps = float(eps)
         self.precision = precision
+        self.method = method
+        self.sigma = sigma
         code = precision_code(precision)
+        methods = {"nufft": 0, "direct": 1}
+        if method not in methods:
+            raise ValueError("method must be 'nufft' or 'direct'")
         lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(
             lib.fresco_nuwnu_create(
                 _lib.dptr(self.radii), self.N, self.ndim, _lib.dptr(self.K), len(self.K), _lib.dptr(self.Sk),
-                _lib.dptr(self.V), len(self.V), _lib.dptr(self.L), code, ctypes.byref(h),
+                _lib.dptr(self.V), len(self.V), _lib.dptr(self.L), self.eps, code, methods[method],
+                float(sigma or 0.0), ctypes.byref(h),
             )
         )
         self._init_handle(h)
@@ -52,8 +59,9 @@ class NUwNU(BasePotential):
         return np.tile(2 * np.pi / self.L, self.N)
 
     def __reduce__(self):
-        return (_rebuild_nuwnu, (self.radii, self.K, self._current("Sk"), self._current("V"), self.L, self.eps, self.precision))
+        return (_rebuild_nuwnu, (self.radii, self.K, self._current("Sk"), self._current("V"), self.L, self.eps,
+                                 self.precision, self.method, self.sigma))
 
 
-def _rebuild_nuwnu(radii, K, Sk, V, L, eps, precision):
-    return NUwNU(radii, K, Sk, V, L, eps, precision=precision)
+def _rebuild_nuwnu(radii, K, Sk, V, L, eps, precision, method, sigma):
+    return NUwNU(radii, K, Sk, V, L, eps, precision=precision, method=method, sigma=sigma)

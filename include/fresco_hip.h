@@ -109,19 +109,23 @@ int fresco_uwnu_create(const int64_t* shape, int d, const double* K, int64_t nkd
                        double sigma, fresco_pot** out);
 
 /*
- * NUwNU: non-uniform points, non-uniform k vectors (exact sums, float64 phases).
+ * NUwNU: non-uniform points, non-uniform k vectors.
  * Replaces fresco::NUwNU::NUwNU (source/fresco/nuwnu.hpp:36-79) as constructed
  * by fresco/potentials/nuwnu.pyx:10-23.
  *   radii[n]      point radii (weights as for NUwU)
  *   K[nkd]        k vectors interleaved, unscaled (as in the reference); Nk = nkd / d
  *   Sk[m], V[m]   m >= Nk
- *   precision     FRESCO_PREC_FP32_ADJOINT = FP32 sin/cos accumulation in
- *                 blocks (float64 sums across blocks), otherwise FP64
+ *   eps, sigma    type-3 NUFFT tolerance and upsampling (0 = 2.0)
+ *   precision     FRESCO_PREC_FP32_ADJOINT = FP32 grids / sums, otherwise FP64
+ *   method        FRESCO_NUWNU_TYPE3 (the reference's FINUFFT type 3, nuwnu.hpp:77-78)
+ *                 or FRESCO_NUWNU_DIRECT (exact sums with float64 phases)
  * ndof = n * d.
  */
+#define FRESCO_NUWNU_TYPE3 0
+#define FRESCO_NUWNU_DIRECT 1
 int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, int64_t nkd,
-                        const double* Sk, const double* V, int64_t m, const double* L, int precision,
-                        fresco_pot** out);
+                        const double* Sk, const double* V, int64_t m, const double* L, double eps,
+                        int precision, int method, double sigma, fresco_pot** out);
 
 /* pair interactions and distance methods for fresco_pair_create
  * (method values = fresco.distances.Distance, distance_enum.py) */

@@ -207,7 +207,7 @@ def test_nuwnu_target_grads(mods):
     Sk[::5] = 0
     V = rng.uniform(0, 1, nk)
     p = rng.uniform(0, 1, N * d)
-    pot = P.NUwNU(np.ones(N), K.ravel(), Sk, V, np.ones(d))
+    pot = P.NUwNU(np.ones(N), K.ravel(), Sk, V, np.ones(d), method="direct")
     S0 = torch.tensor(Sk, requires_grad=True)
     Vt = torch.tensor(V, requires_grad=True)
     x = torch.tensor(p, requires_grad=True)

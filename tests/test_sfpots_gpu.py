@@ -137,12 +137,14 @@ def test_uwnu_integer_k_is_a_dft(P):
 # NUwNU
 # ---------------------------------------------------------------------------
 
-NUNU_TOL = {"fp64": 1e-11, "fp32": 1e-5}
+# (method, precision): (eps, tolerance); "direct" sums are exact up to rounding
+NUNU_CASES = {("direct", "fp64"): (1e-6, 1e-11), ("direct", "fp32"): (1e-6, 1e-5),
+              ("nufft", "fp64"): (1e-12, 1e-9), ("nufft", "fp32"): (1e-6, 1e-5)}
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("method,prec", sorted(NUNU_CASES))
 @pytest.mark.parametrize("d,N,nk", [(1, 300, 90), (2, 500, 300), (3, 700, 400)])
-def test_nuwnu_parity(P, prec, d, N, nk):
+def test_nuwnu_parity(P, method, prec, d, N, nk):
     rng = np.random.default_rng(100 + d)
     radii = rng.uniform(0.5, 1.5, N)
     K = rng.uniform(-20, 20, (nk, d))
@@ -153,13 +155,14 @@ def test_nuwnu_parity(P, prec, d, N, nk):
     p = rng.uniform(-0.4, 1.4, N * d)
     ref = O.NUwNU(radii, K.ravel(), Sk, V, L)
     E0, g0 = ref.get_energy_gradient(p)
-    pot = P.NUwNU(radii, K.ravel(), Sk, V, L, precision=prec)
+    eps, tol = NUNU_CASES[(method, prec)]
+    pot = P.NUwNU(radii, K.ravel(), Sk, V, L, eps, precision=prec, method=method)
     E, g = pot.get_energy_gradient(p)
-    print("nuwnu", d, N, nk, prec, E, E0, flush=True)
-    assert abs(E - E0) / abs(E0) < NUNU_TOL[prec]
-    assert rel(g, g0) < NUNU_TOL[prec]
-    assert rel(pot.get_rho(), ref.rho(p)) < NUNU_TOL[prec]
-    assert abs(pot.get_energy(p) - E0) / abs(E0) < NUNU_TOL[prec]
+    print("nuwnu", method, d, N, nk, prec, pot.plan_info(), E, E0, flush=True)
+    assert abs(E - E0) / abs(E0) < tol
+    assert rel(g, g0) < tol
+    assert rel(pot.get_rho(), ref.rho(p)) < tol
+    assert abs(pot.get_energy(p) - E0) / abs(E0) < tol
 
 
 def test_nuwnu_mode_count_follows_k_and_sizes_checked(P):
@@ -172,3 +175,27 @@ def test_nuwnu_mode_count_follows_k_and_sizes_checked(P):
         P.UwNU(np.array([4, 4]), K.ravel(), np.ones(7), np.ones(7), np.ones(2))
     with pytest.raises(ValueError):
         P.UwU(np.array([4, 4]), 0, np.ones(15), np.ones(15), np.ones(2))
+
+
+def test_nuwnu_type3_offcentre_k_and_fd(P):
+    """k box far from the origin (centre D != 0) and unequal box lengths:
+    the type-3 centring phases; fp64 gradient against central differences."""
+    rng = np.random.default_rng(77)
+    d, N, nk = 2, 400, 250
+    K = rng.uniform(30, 55, (nk, d)) * np.array([1.0, -1.0])
+    Sk = rng.uniform(0.5, 2, nk)
+    V = rng.uniform(0, 1, nk)
+    L = np.array([1.0, 1.3])
+    p = rng.uniform(0, 1.3, N * d)
+    radii = np.ones(N)
+    ref = O.NUwNU(radii, K.ravel(), Sk, V, L)
+    E0, g0 = ref.get_energy_gradient(p)
+    pot = P.NUwNU(radii, K.ravel(), Sk, V, L, 1e-12)
+    E, g = pot.get_energy_gradient(p)
+    assert abs(E - E0) / abs(E0) < 1e-9
+    assert rel(g, g0) < 1e-9
+    v = rng.normal(size=p.size)
+    h = 1e-6
+    fd = (pot.get_energy(p + h * v) - pot.get_energy(p - h * v)) / (2 * h)
+    gp = g * np.tile(2 * math.pi / L, N)
+    assert abs(fd - np.dot(gp, v)) / abs(fd) < 1e-6
