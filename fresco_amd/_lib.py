@@ -40,6 +40,12 @@ SIGNATURES = {
         [c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double, c_double_p, c_double_p,
          ctypes.c_int64, c_double_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, c_void_pp],
     ),
+    "fresco_nuwu_batch_create": (
+        ctypes.c_int,
+        [ctypes.c_int, c_int64_p, c_double_p, ctypes.c_int, c_int64_p, c_double_p, c_double_p, c_double_p,
+         ctypes.c_double, ctypes.c_int, ctypes.c_double, c_void_pp],
+    ),
+    "fresco_pot_pattern_energies": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int]),
     "fresco_pair_create": (
         ctypes.c_int,
         [ctypes.c_int, ctypes.c_double, ctypes.c_double, c_double_p, ctypes.c_int64, ctypes.c_int, c_double_p,

@@ -155,6 +155,29 @@ int fresco_pair_create(int kind, double p1, double eps, const double* radii, int
   });
 }
 
+int fresco_nuwu_batch_create(int npat, const int64_t* npts, const double* radii, int d, const int64_t* nmodes,
+                             const double* Sk, const double* V, const double* L, double eps, int precision,
+                             double sigma, fresco_pot** out) {
+  return guarded([&]() {
+    if (!out) throw fresco_hip::InvalidArgument("out is null");
+    *out = nullptr;
+    if (!npts || !radii || !nmodes || !Sk || !V || !L) throw fresco_hip::InvalidArgument("null array argument");
+    if (precision < 0 || precision > 2) throw fresco_hip::InvalidArgument("unknown precision");
+    if (!(eps > 0)) throw fresco_hip::InvalidArgument("eps must be positive");
+    auto h = std::make_unique<fresco_pot>();
+    h->impl = fresco_hip::make_nuwu_batch(npat, npts, radii, d, nmodes, Sk, V, L, eps, precision, sigma);
+    *out = h.release();
+  });
+}
+
+int fresco_pot_pattern_energies(fresco_pot* pot, double* energies, int npat) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!energies) throw fresco_hip::InvalidArgument("null pointer");
+    p->pattern_energies(energies, npat, nullptr);
+  });
+}
+
 int fresco_combined_create(fresco_pot* const* pots, int count, fresco_pot** out) {
   return guarded([&]() {
     if (!out) throw fresco_hip::InvalidArgument("out is null");

@@ -152,9 +152,12 @@ size_t interp_lds_bytes(const GridGeom& g, int elem_bytes, int w) {
   return (size_t)elem_bytes * srx * ry * rz;
 }
 
-GridGeom make_geom(int d, const int* nf, int w, int elem_bytes) {
+GridGeom make_geom(int d, const int* nf, int w, int elem_bytes, int batch) {
   GridGeom g;
   g.d = d;
+  if (batch < 1) throw InvalidArgument("batch must be positive");
+  if (batch > 1 && d == 3) throw InvalidArgument("batched plans are 1D / 2D");
+  g.batch = batch;
   for (int a = 0; a < 3; ++a) g.nf[a] = a < d ? nf[a] : 1;
   for (int a = 0; a < 3; ++a) g.tile[a] = 1;
   if (d == 3) {
@@ -190,13 +193,14 @@ GridGeom make_geom(int d, const int* nf, int w, int elem_bytes) {
     while (interp_lds_bytes(g, elem_bytes, w) > 80 * 1024 || spread_lds_bytes(g, elem_bytes, w) > 40 * 1024)
       if (!halve_largest(4)) break;
     const int minsz = d == 2 ? 16 : 64;
-    while (ntiles() < 1024)
+    while (ntiles() * batch < 1024)
       if (!halve_largest(minsz)) break;
     clamp();
   }
   for (int a = 0; a < 3; ++a) g.ntile[a] = (g.nf[a] + g.tile[a] - 1) / g.tile[a];
   g.nbins = g.ntile[0] * g.ntile[1] * g.ntile[2];
-  g.key_bits = ceil_log2(g.nbins + 1);
+  if ((long long)g.nbins * batch >= (1LL << 31)) throw InvalidArgument("too many bins");
+  g.key_bits = ceil_log2(g.nbins * batch + 1);
   g.cell_bits = d == 3 ? 10 : (d == 2 ? 16 : 31);
   for (int a = 0; a < d; ++a)
     if (g.nf[a] > (1 << g.cell_bits) - 1)
@@ -233,10 +237,10 @@ __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, 
 
 template <typename K>
 __global__ void k_bin_keys(const double* __restrict__ x, int64_t n, MapDev m, K* __restrict__ key,
-                           uint32_t* __restrict__ idx) {
+                           uint32_t* __restrict__ idx, const uint32_t* __restrict__ pat, uint32_t nbins) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  uint32_t b = 0, mul = 1;
+  uint32_t b = pat ? pat[j] * nbins : 0u, mul = 1;
   for (int a = 0; a < m.d; ++a) {
     int cell;
     double off;
@@ -313,10 +317,11 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   es = e;
   n = npts;
   // 16-bit bin keys when they fit: rocPRIM then runs its onesweep radix sort
-  small_keys = g.nbins + 1 < 65535;
+  const int64_t tb = (int64_t)g.nbins * g.batch;
+  small_keys = tb + 1 < 65535;
   idx_in.allocate(n);
   idx.allocate(n);
-  bin_start.allocate(g.nbins + 1);
+  bin_start.allocate(tb + 1);
   scell.allocate(n);
   rec.allocate(n);
   size_t bytes = 0;
@@ -336,26 +341,28 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
 }
 
 template <typename T>
-void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipStream_t s) {
+void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipStream_t s, const uint32_t* pat) {
   MapDev m = make_map_dev(geom, es, cm);
   const unsigned thr = 256;
   size_t bytes = sort_tmp_bytes;
+  if ((geom.batch > 1) != (pat != nullptr)) throw InvalidArgument("pattern indices needed exactly for batched plans");
+  const int tb = geom.nbins * geom.batch;
   if (small_keys) {
     hipLaunchKernelGGL(k_bin_keys<uint16_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key16.get(),
-                       idx_in.get());
+                       idx_in.get(), pat, (uint32_t)geom.nbins);
     FRESCO_HIP_CHECK(hipGetLastError());
     FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key16.get(), key16_sorted.get(),
                                                         idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
-    hipLaunchKernelGGL(k_bin_start<uint16_t>, dim3(blocks_for(geom.nbins + 1, thr)), dim3(thr), 0, s,
-                       key16_sorted.get(), n, geom.nbins, bin_start.get());
+    hipLaunchKernelGGL(k_bin_start<uint16_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key16_sorted.get(), n,
+                       tb, bin_start.get());
   } else {
     hipLaunchKernelGGL(k_bin_keys<uint32_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(),
-                       idx_in.get());
+                       idx_in.get(), pat, (uint32_t)geom.nbins);
     FRESCO_HIP_CHECK(hipGetLastError());
     FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key.get(), key_sorted.get(),
                                                         idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
-    hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(geom.nbins + 1, thr)), dim3(thr), 0, s,
-                       key_sorted.get(), n, geom.nbins, bin_start.get());
+    hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key_sorted.get(), n,
+                       tb, bin_start.get());
   }
   FRESCO_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, idx.get(), c, scell.get(),
@@ -472,6 +479,9 @@ __global__ void __launch_bounds__(64)
   constexpr int SH = Lanes<WMAX>::SH;
   constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
 
+  // batched plans: pattern blockIdx.y has its own bins and grid
+  bin_start += (int64_t)blockIdx.y * a.ntile[0] * a.ntile[1] * a.ntile[2];
+  grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1] * a.nf[2] * NC;
   const int b = blockIdx.x;
   int t[3];
   t[0] = b % a.ntile[0];
@@ -779,7 +789,7 @@ static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* gr
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
   ProfScope ps(ST_SPREAD, s);
-  hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins), dim3(64), sp.lds_bytes, s, p.scell.get(), p.rec.get(),
+  hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins, g.batch), dim3(64), sp.lds_bytes, s, p.scell.get(), p.rec.get(),
                      p.bin_start.get(), sp.cand.get(), a, grid);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
@@ -875,6 +885,8 @@ __global__ void __launch_bounds__(256)
   constexpr int SH = Lanes<WMAX>::SH;
   constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
 
+  bin_start += (int64_t)blockIdx.y * a.ntile[0] * a.ntile[1] * a.ntile[2];  // batched plans
+  grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1] * a.nf[2];
   const int b = blockIdx.x;
   const int beg = bin_start[b];
   const int end = bin_start[b + 1];
@@ -1725,7 +1737,7 @@ static void launch_interp(const SortedPoints<T>& p, const T* grid, bool deriv, c
   a.k.two_over_w = (T)(2.0 / p.es.w);
   size_t lds = interp_lds_bytes(g, (int)sizeof(T), a.w);
   if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS; use a smaller tile");
-  dim3 grd(g.nbins), blk(256);
+  dim3 grd(g.nbins, g.batch), blk(256);
   if (deriv)
     hipLaunchKernelGGL((k_interp<T, D, WMAX, true>), grd, blk, lds, s, p.scell.get(), p.rec.get(), p.idx.get(),
                        p.bin_start.get(), grid, a, out);
@@ -1893,7 +1905,7 @@ FftPlan::~FftPlan() {
   if (info_) rocfft_execution_info_destroy(info_);
 }
 
-void FftPlan::create(Kind k, bool dbl, int d, const int* len) {
+void FftPlan::create(Kind k, bool dbl, int d, const int* len, int batch) {
   rocfft_init_once();
   size_t lengths[3];
   for (int a = 0; a < d; ++a) lengths[a] = (size_t)len[a];
@@ -1902,8 +1914,8 @@ void FftPlan::create(Kind k, bool dbl, int d, const int* len) {
                              : k == C2C_FWD ? rocfft_transform_type_complex_forward
                                             : rocfft_transform_type_complex_inverse;
   rocfft_check(rocfft_plan_create(&plan_, rocfft_placement_notinplace, tt,
-                                  dbl ? rocfft_precision_double : rocfft_precision_single, (size_t)d, lengths, 1,
-                                  nullptr),
+                                  dbl ? rocfft_precision_double : rocfft_precision_single, (size_t)d, lengths,
+                                  (size_t)std::max(1, batch), nullptr),
                "plan_create");
   size_t wb = 0;
   rocfft_check(rocfft_plan_get_work_buffer_size(plan_, &wb), "get_work_buffer_size");

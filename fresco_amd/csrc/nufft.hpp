@@ -56,7 +56,8 @@ struct GridGeom {
   int nf[3] = {1, 1, 1};     // fine grid cells per dimension
   int tile[3] = {1, 1, 1};   // spreader tile (= bin) size per dimension
   int ntile[3] = {1, 1, 1};  // tiles per dimension
-  int nbins = 1;
+  int nbins = 1;   // bins per pattern
+  int batch = 1;   // independent patterns, each with its own grid and bins
   int key_bits = 1;
   int cell_bits = 10;  // bits per dimension of the packed first-cell word
   int64_t ncells() const { return int64_t(nf[0]) * nf[1] * nf[2]; }
@@ -64,7 +65,7 @@ struct GridGeom {
 
 // Choose tiles for a fine grid (power-of-two tiles, enough of them to fill
 // the GPU, LDS budget respected).
-GridGeom make_geom(int d, const int* nf, int w, int elem_bytes);
+GridGeom make_geom(int d, const int* nf, int w, int elem_bytes, int batch = 1);
 int lds_row_stride(int n, int elem_bytes, int w);
 size_t spread_lds_bytes(const GridGeom& g, int elem_bytes, int w);
 size_t interp_lds_bytes(const GridGeom& g, int elem_bytes, int w);
@@ -75,8 +76,9 @@ class SortedPoints {
  public:
   using R4 = typename Vec4<T>::type;
   void init(const GridGeom& g, const EsParams& es, int64_t n);
-  // x: device float64 [n*d]; c: device T[n] weights (may be null -> 1).
-  void set(const double* x, const CoordMap& m, const T* c, hipStream_t s);
+  // x: device float64 [n*d]; c: device T[n] weights (may be null -> 1);
+  // pat: device pattern index of every point (geom.batch > 1), else null.
+  void set(const double* x, const CoordMap& m, const T* c, hipStream_t s, const uint32_t* pat = nullptr);
 
   int64_t n = 0;
   GridGeom geom;
@@ -84,7 +86,7 @@ class SortedPoints {
   DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
   DeviceBuffer<uint16_t> key16, key16_sorted;            // used when nbins < 65535 (onesweep sort)
   bool small_keys = false;
-  DeviceBuffer<int> bin_start;                          // [nbins + 1]
+  DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
   DeviceBuffer<unsigned char> sort_tmp;
@@ -142,7 +144,8 @@ class FftPlan {
   ~FftPlan();
   FftPlan(const FftPlan&) = delete;
   FftPlan& operator=(const FftPlan&) = delete;
-  void create(Kind k, bool dbl, int d, const int* lengths_fastest_first);
+  // batch > 1: that many transforms, packed one after another
+  void create(Kind k, bool dbl, int d, const int* lengths_fastest_first, int batch = 1);
   void exec(void* in, void* out, hipStream_t s);
 
  private:

@@ -87,8 +87,14 @@ __global__ void __launch_bounds__(256)
       const double xr = hr * ip, xi = -hi * ip;
       const int h = pd.h[i], hn = pd.hneg[i];
       if (MODE == 1) {
-        X0[h] = Cplx<T>{(T)xr, (T)xi};
-        if (hn >= 0) X0[hn] = Cplx<T>{(T)xr, (T)(-xi)};
+        // k = 0 (the only self-paired mode) moves no point: leaving it out
+        // keeps its constant, of size N, off the grid whose derivative is
+        // interpolated (the constant's derivative is not exactly zero on a
+        // finite kernel)
+        if (!self) {
+          X0[h] = Cplx<T>{(T)xr, (T)xi};
+          if (hn >= 0) X0[hn] = Cplx<T>{(T)xr, (T)(-xi)};
+        }
       } else {
         // f_a = -i k_a factor  =>  X_a = i k_a X
         Cplx<T>* Xs[3] = {X0, X1, X2};

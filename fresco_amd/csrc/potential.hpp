@@ -43,6 +43,14 @@ class Potential {
     (void)s;
     throw RuntimeFailure("this potential has no S0 / V targets");
   }
+  // energy of every pattern of a batched potential (host, n entries) at the
+  // last evaluation
+  virtual void pattern_energies(double* host, int n, hipStream_t s) {
+    (void)host;
+    (void)n;
+    (void)s;
+    throw RuntimeFailure("not a batched potential");
+  }
   // spreader configuration in use: mode 0 register tiles / owner tiles,
   // 1 halo tiles; parts = point-range slices per bin (-1 when not applicable)
   virtual void spread_config(int* mode, int* parts) const {
@@ -113,5 +121,11 @@ std::unique_ptr<Potential> make_nuwnu(const double* radii, int64_t n, int d, con
 // cartesian, 2 periodic (box[d]); cells != 0: cell lists (periodic only)
 std::unique_ptr<Potential> make_pair(int kind, double p1, double eps, const double* radii, int64_t n, int d,
                                      const double* box, int method, int cells, double ncellx_scale);
+
+// npat independent 1D / 2D NUwU patterns: npts[p] points (radii concatenated),
+// nmodes[p] = (2K_p+1)^d modes (Sk, V concatenated), one box L
+std::unique_ptr<Potential> make_nuwu_batch(int npat, const int64_t* npts, const double* radii, int d,
+                                           const int64_t* nmodes, const double* Sk, const double* V, const double* L,
+                                           double eps, int precision, double sigma);
 
 }  // namespace fresco_hip

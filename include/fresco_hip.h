@@ -127,6 +127,27 @@ int fresco_nuwnu_create(const double* radii, int64_t n, int d, const double* K, 
                         const double* Sk, const double* V, int64_t m, const double* L, double eps,
                         int precision, int method, double sigma, fresco_pot** out);
 
+/*
+ * Batch of npat independent NUwU patterns (1D / 2D), evaluated together: one
+ * sort, one spreading and one interpolation launch over every pattern's bins,
+ * batched FFTs.  Each pattern follows NUwU::NUwU (source/fresco/nuwu.hpp:35-52):
+ *   npts[p]          points of pattern p; radii[sum npts] concatenated
+ *   nmodes[p]        (2K_p+1)^d modes of pattern p; Sk, V [sum nmodes] concatenated
+ *   L[d]             box (shared)
+ *   precision        FRESCO_PREC_FP32_ADJOINT or FRESCO_PREC_FP64_ADJOINT (the
+ *                    one-transform adjoint gradient); FP64_PLAIN maps to FP64_ADJOINT
+ * x / grad: the patterns' coordinate vectors concatenated (ndof = d sum npts);
+ * the energy is the sum of the patterns' energies.  No reference counterpart
+ * (the reference evaluates one pattern per object).
+ */
+int fresco_nuwu_batch_create(int npat, const int64_t* npts, const double* radii, int d,
+                             const int64_t* nmodes, const double* Sk, const double* V, const double* L,
+                             double eps, int precision, double sigma, fresco_pot** out);
+
+/* Energy of every pattern of a batched potential at its last evaluation
+ * (host double[npat]). */
+int fresco_pot_pattern_energies(fresco_pot* pot, double* energies, int npat);
+
 /* pair interactions and distance methods for fresco_pair_create
  * (method values = fresco.distances.Distance, distance_enum.py) */
 #define FRESCO_PAIR_INVERSE_POWER 0
