@@ -217,8 +217,9 @@ def measured_traffic(stage, info):
     try:
         with open(path) as f:
             for line in f:
-                if line.strip().startswith(key):
-                    return int(float(line.split(":", 1)[1].strip()))
+                line = line.strip()
+                if line.startswith(key):
+                    return int(float(line[len(key):].strip()))
     except OSError:
         pass
     return None

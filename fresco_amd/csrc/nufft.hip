@@ -1,7 +1,6 @@
 // Non-uniform FFT kernels for gfx950: point binning/sorting, owner-computes
 // LDS-tiled spreading, wavefront-per-point interpolation, rocFFT plans.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
@@ -318,52 +317,28 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   n = npts;
   // 16-bit bin keys when they fit: rocPRIM then runs its onesweep radix sort
   const int64_t tb = (int64_t)g.nbins * g.batch;
-  small_keys = tb + 1 < 65535;
   idx_in.allocate(n);
   idx.allocate(n);
   bin_start.allocate(tb + 1);
   scell.allocate(n);
   rec.allocate(n);
-  size_t bytes = 0;
-  if (small_keys) {
-    key16.allocate(n);
-    key16_sorted.allocate(n);
-    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key16.get(), key16_sorted.get(),
-                                                        idx_in.get(), idx.get(), (int)n, 0, g.key_bits));
-  } else {
-    key.allocate(n);
-    key_sorted.allocate(n);
-    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.get(), key_sorted.get(), idx_in.get(),
-                                                        idx.get(), (int)n, 0, g.key_bits));
-  }
-  sort_tmp_bytes = std::max<size_t>(bytes, 16);
-  sort_tmp.allocate(sort_tmp_bytes);
+  key.allocate(n);
+  key_sorted.allocate(n);
+  sorter.init(n, g.key_bits);
 }
 
 template <typename T>
 void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipStream_t s, const uint32_t* pat) {
   MapDev m = make_map_dev(geom, es, cm);
   const unsigned thr = 256;
-  size_t bytes = sort_tmp_bytes;
   if ((geom.batch > 1) != (pat != nullptr)) throw InvalidArgument("pattern indices needed exactly for batched plans");
   const int tb = geom.nbins * geom.batch;
-  if (small_keys) {
-    hipLaunchKernelGGL(k_bin_keys<uint16_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key16.get(),
-                       idx_in.get(), pat, (uint32_t)geom.nbins);
-    FRESCO_HIP_CHECK(hipGetLastError());
-    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key16.get(), key16_sorted.get(),
-                                                        idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
-    hipLaunchKernelGGL(k_bin_start<uint16_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key16_sorted.get(), n,
-                       tb, bin_start.get());
-  } else {
-    hipLaunchKernelGGL(k_bin_keys<uint32_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(),
-                       idx_in.get(), pat, (uint32_t)geom.nbins);
-    FRESCO_HIP_CHECK(hipGetLastError());
-    FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.get(), bytes, key.get(), key_sorted.get(),
-                                                        idx_in.get(), idx.get(), (int)n, 0, geom.key_bits, s));
-    hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key_sorted.get(), n,
-                       tb, bin_start.get());
-  }
+  hipLaunchKernelGGL(k_bin_keys<uint32_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(),
+                     idx_in.get(), pat, (uint32_t)geom.nbins);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  sorter.sort(key.get(), idx_in.get(), key_sorted.get(), idx.get(), s);
+  hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key_sorted.get(), n, tb,
+                     bin_start.get());
   FRESCO_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, idx.get(), c, scell.get(),
                      rec.get());

@@ -18,6 +18,7 @@
 
 #include "common.hpp"
 #include "es_kernel.hpp"
+#include "sort.hpp"
 
 struct rocfft_plan_t;
 struct rocfft_execution_info_t;
@@ -84,13 +85,10 @@ class SortedPoints {
   GridGeom geom;
   EsParams es;
   DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
-  DeviceBuffer<uint16_t> key16, key16_sorted;            // used when nbins < 65535 (onesweep sort)
-  bool small_keys = false;
+  BinSort sorter;                                       // stable LSD radix sort, 8-bit digits
   DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
-  DeviceBuffer<unsigned char> sort_tmp;
-  size_t sort_tmp_bytes = 0;
 };
 
 template <typename T>

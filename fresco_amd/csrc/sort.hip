@@ -1,0 +1,203 @@
+// Stable LSD radix sort of (bin key, point index) pairs with digits of up to
+// 11 bits (keys below 2^11, such as the bins of the 3D band grids, take one
+// pass).
+//
+// Per digit pass:
+//   k_rs_upsweep   digit counts of every 1024-key tile (LDS histogram)
+//   k_rs_scan_seg  exclusive scan of the digit-major count table in
+//                  segments of 4096 entries (one workgroup each, coalesced)
+//   k_rs_scan_top  exclusive scan of the segment totals (one workgroup)
+//   k_rs_scatter   every tile moves its pairs to their places in index
+//                  order: 256 keys per round, each wavefront ranks its lanes
+//                  among equal digits with one ballot per digit bit (lanes
+//                  with the same digit = the AND of the matching bit masks),
+//                  and per-wavefront counts in LDS order the wavefronts.
+// Equal keys keep their input order, so the result equals any stable sort's;
+// no launch records a memory fill, so the sort can sit in a captured graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "sort.hpp"
+
+namespace fresco_hip {
+
+namespace {
+
+constexpr int RS_T = 256;                  // threads per tile workgroup
+constexpr int RS_ROUNDS = 4;               // rounds of 256 keys per tile
+constexpr int RS_TILE = RS_T * RS_ROUNDS;  // keys per tile
+constexpr int RS_MAXB = 2048;              // buckets of an 11-bit digit
+constexpr int RS_SEG = 4096;               // scan segment
+
+__global__ void __launch_bounds__(RS_T)
+    k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t dmask, int nbuck,
+                 uint32_t* __restrict__ cnt, int nblk) {
+  __shared__ uint32_t h[RS_MAXB];
+  const int t = threadIdx.x;
+  for (int d = t; d < nbuck; d += RS_T) h[d] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const int64_t e = base + (int64_t)r * RS_T + t;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & dmask], 1u);
+  }
+  __syncthreads();
+  for (int d = t; d < nbuck; d += RS_T) cnt[(int64_t)d * nblk + blockIdx.x] = h[d];
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  // 256 threads: Hillis-Steele in LDS
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < RS_T; off <<= 1) {
+    const uint32_t a = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  total = sh[RS_T - 1];
+  const uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+// segment s = entries [s * RS_SEG, (s+1) * RS_SEG): exclusive scan in place,
+// the segment total to tops[s]; thread t owns 16 consecutive entries
+__global__ void __launch_bounds__(RS_T) k_rs_scan_seg(uint32_t* __restrict__ c, int64_t m, uint32_t* __restrict__ tops) {
+  __shared__ uint32_t sh[RS_T];
+  constexpr int PER = RS_SEG / RS_T;
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * RS_SEG + (int64_t)t * PER;
+  uint32_t v[PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    v[i] = base + i < m ? c[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, sh, total);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    if (base + i < m) c[base + i] = run;
+    run += v[i];
+  }
+  if (t == 0) tops[blockIdx.x] = total;
+}
+
+// exclusive scan of the segment totals (nseg entries), one workgroup
+__global__ void __launch_bounds__(RS_T) k_rs_scan_top(uint32_t* __restrict__ tops, int nseg) {
+  __shared__ uint32_t sh[RS_T];
+  const int t = threadIdx.x;
+  const int per = (nseg + RS_T - 1) / RS_T;
+  const int lo = t * per < nseg ? t * per : nseg;
+  const int hi = lo + per < nseg ? lo + per : nseg;
+  uint32_t s = 0;
+  for (int i = lo; i < hi; ++i) s += tops[i];
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, sh, total);
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t a = tops[i];
+    tops[i] = run;
+    run += a;
+  }
+}
+
+__global__ void __launch_bounds__(RS_T)
+    k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
+                 uint32_t dmask, int dbits, int nbuck, const uint32_t* __restrict__ offs,
+                 const uint32_t* __restrict__ tops, int nblk, uint32_t* __restrict__ kout,
+                 uint32_t* __restrict__ vout) {
+  __shared__ uint32_t run[RS_MAXB];
+  __shared__ uint32_t wc[RS_T / 64][RS_MAXB];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  for (int d = t; d < nbuck; d += RS_T) {
+    const int64_t i = (int64_t)d * nblk + blockIdx.x;
+    run[d] = offs[i] + tops[i / RS_SEG];
+  }
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    for (int d = t; d < nbuck; d += RS_T) {
+#pragma unroll
+      for (int w = 0; w < RS_T / 64; ++w) wc[w][d] = 0;
+    }
+    __syncthreads();
+    const int64_t e = base + (int64_t)r * RS_T + t;
+    const bool valid = e < n;
+    const uint32_t k = valid ? kin[e] : 0u;
+    const uint32_t dg = (k >> shift) & dmask;
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+      const bool bit = (dg >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const int rank = __popcll(same & lt);
+    if (valid && rank == 0) wc[wv][dg] = (uint32_t)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[dg] + (uint32_t)rank;
+      for (int w = 0; w < wv; ++w) pos += wc[w][dg];
+      kout[pos] = k;
+      vout[pos] = vin[e];
+    }
+    __syncthreads();
+    for (int d = t; d < nbuck; d += RS_T) {
+      uint32_t add = 0;
+#pragma unroll
+      for (int w = 0; w < RS_T / 64; ++w) add += wc[w][d];
+      run[d] += add;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+void BinSort::init(int64_t n, int key_bits) {
+  if (n <= 0 || n >= (1LL << 31)) throw InvalidArgument("radix sort size out of range");
+  n_ = n;
+  key_bits = std::max(1, key_bits);
+  passes_ = (key_bits + 10) / 11;
+  dbits_ = (key_bits + passes_ - 1) / passes_;
+  nblk_ = (int)((n + RS_TILE - 1) / RS_TILE);
+  const int64_t m = ((int64_t)1 << dbits_) * nblk_;
+  nseg_ = (int)((m + RS_SEG - 1) / RS_SEG);
+  cnt_.allocate((size_t)m);
+  tops_.allocate((size_t)nseg_);
+  ktmp_.allocate(n);
+  vtmp_.allocate(n);
+}
+
+void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out,
+                   hipStream_t s) {
+  const uint32_t* kin = keys;
+  const uint32_t* vin = vals;
+  const int nbuck = 1 << dbits_;
+  const uint32_t dmask = (uint32_t)nbuck - 1u;
+  const int64_t m = (int64_t)nbuck * nblk_;
+  for (int p = 0; p < passes_; ++p) {
+    // the last pass writes the outputs; earlier ones alternate so that no
+    // pass reads and writes the same buffers
+    const bool to_out = ((passes_ - 1 - p) % 2) == 0;
+    uint32_t* ko = to_out ? keys_out : ktmp_.get();
+    uint32_t* vo = to_out ? vals_out : vtmp_.get();
+    const int shift = dbits_ * p;
+    hipLaunchKernelGGL(k_rs_upsweep, dim3(nblk_), dim3(RS_T), 0, s, kin, n_, shift, dmask, nbuck, cnt_.get(), nblk_);
+    hipLaunchKernelGGL(k_rs_scan_seg, dim3(nseg_), dim3(RS_T), 0, s, cnt_.get(), m, tops_.get());
+    hipLaunchKernelGGL(k_rs_scan_top, dim3(1), dim3(RS_T), 0, s, tops_.get(), nseg_);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nblk_), dim3(RS_T), 0, s, kin, vin, n_, shift, dmask, dbits_, nbuck,
+                       cnt_.get(), tops_.get(), nblk_, ko, vo);
+    FRESCO_HIP_CHECK(hipGetLastError());
+    kin = ko;
+    vin = vo;
+  }
+}
+
+}  // namespace fresco_hip

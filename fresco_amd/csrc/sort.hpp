@@ -1,0 +1,29 @@
+// Stable radix sort of uint32 (key, value) pairs on the device, built from
+// kernel launches only (no memory fills), so it can be captured in a graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace fresco_hip {
+
+class BinSort {
+ public:
+  // n pairs; keys are below 2^key_bits
+  void init(int64_t n, int key_bits);
+  // keys / vals -> keys_out / vals_out (stable; inputs are not modified)
+  void sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s);
+
+ private:
+  int64_t n_ = 0;
+  int passes_ = 1;
+  int dbits_ = 1;  // digit width, at most 11 bits
+  int nblk_ = 1;
+  int nseg_ = 1;
+  DeviceBuffer<uint32_t> cnt_, tops_, ktmp_, vtmp_;
+};
+
+}  // namespace fresco_hip
