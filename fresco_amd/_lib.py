@@ -104,6 +104,7 @@ SIGNATURES = {
         [ctypes.c_void_p, c_double_p, c_double_p, c_int_p, c_int_p, c_int_p, c_double_p],
     ),
     "fresco_opt_get_x": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    "fresco_opt_graph_stats": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p]),
     "fresco_opt_get_g": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),
@@ -117,6 +118,7 @@ SIGNATURES = {
 N_STAGES = 8
 
 OPT_TOL, OPT_MAXSTEP, OPT_MAX_ITER, OPT_IPRINT, OPT_VERBOSITY, OPT_H0, OPT_MAX_F_RISE, OPT_USE_RELATIVE_F = range(8)
+OPT_USE_GRAPH = 8
 
 EG_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, c_double_p, c_double_p, c_double_p, ctypes.c_void_p)
 

@@ -321,33 +321,20 @@ class BatchNUwU final : public Potential {
     forward(x, map_e_, s);
     hipLaunchKernelGGL((k_batch_loss<T, 0>), dim3(nblk_), dim3(BT), 0, s, S_.get(), pd_, X_.get(), partial_.get());
     FRESCO_HIP_CHECK(hipGetLastError());
-    finish_energy(s);
+    finish_energy(s, e_.get());
     return host_.fetch(e_.get(), s);
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
-    forward(x, map_g_, s);
-    {
-      ProfScope p(ST_ZERO, s);
-      fill_zero(X_.get(), (size_t)nspec_ * P_, s);
-    }
-    {
-      ProfScope p(ST_LOSS, s);
-      hipLaunchKernelGGL((k_batch_loss<T, 1>), dim3(nblk_), dim3(BT), 0, s, S_.get(), pd_, X_.get(),
-                         partial_.get());
-      FRESCO_HIP_CHECK(hipGetLastError());
-      finish_energy(s);
-    }
-    {
-      ProfScope p(ST_FFT_INV, s);
-      c2r_.exec(X_.get(), grid_.get(), s);
-    }
-    {
-      ProfScope p(ST_INTERP, s);
-      interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
-    }
+    enqueue_eg(x, grad, e_.get(), s);
     return host_.fetch(e_.get(), s);
   }
+
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return spreader_.tuned; }
 
   void pattern_energies(double* host, int n, hipStream_t s) override {
     if (n != P_) throw InvalidArgument("expected " + std::to_string(P_) + " patterns");
@@ -365,6 +352,29 @@ class BatchNUwU final : public Potential {
   }
 
  private:
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
+    forward(x, map_g_, s);
+    {
+      ProfScope p(ST_ZERO, s);
+      fill_zero(X_.get(), (size_t)nspec_ * P_, s);
+    }
+    {
+      ProfScope p(ST_LOSS, s);
+      hipLaunchKernelGGL((k_batch_loss<T, 1>), dim3(nblk_), dim3(BT), 0, s, S_.get(), pd_, X_.get(),
+                         partial_.get());
+      FRESCO_HIP_CHECK(hipGetLastError());
+      finish_energy(s, e_out);
+    }
+    {
+      ProfScope p(ST_FFT_INV, s);
+      c2r_.exec(X_.get(), grid_.get(), s);
+    }
+    {
+      ProfScope p(ST_INTERP, s);
+      interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
+    }
+  }
+
   void forward(const double* x, const CoordMap& m, hipStream_t s) {
     {
       ProfScope p(ST_SORT, s);
@@ -375,10 +385,10 @@ class BatchNUwU final : public Potential {
     r2c_.exec(grid_.get(), S_.get(), s);
   }
 
-  void finish_energy(hipStream_t s) {
+  void finish_energy(hipStream_t s, double* e_out) {
     hipLaunchKernelGGL(k_pattern_sums, dim3((P_ + 127) / 128), dim3(128), 0, s, partial_.get(), pblk0_.get(), P_,
-                       epat_.get(), e_.get());
-    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, epat_.get(), P_, e_.get());
+                       epat_.get(), e_out);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, epat_.get(), P_, e_out);
     FRESCO_HIP_CHECK(hipGetLastError());
   }
 

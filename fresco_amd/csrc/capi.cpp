@@ -240,6 +240,7 @@ int fresco_opt_set_param(fresco_opt* opt, int param, double value) {
       case FRESCO_OPT_MAX_ITER: o->set_max_iter((int)value); break;
       case FRESCO_OPT_IPRINT: o->set_iprint((int)value); break;
       case FRESCO_OPT_VERBOSITY: o->set_verbosity((int)value); break;
+      case FRESCO_OPT_USE_GRAPH: o->set_use_graph(value != 0.0); break;
       case FRESCO_OPT_H0:
       case FRESCO_OPT_MAX_F_RISE:
       case FRESCO_OPT_USE_RELATIVE_F:
@@ -289,6 +290,13 @@ int fresco_opt_set_func_gradient(fresco_opt* opt, double f, const double* grad, 
 
 int fresco_opt_stop_criterion(fresco_opt* opt, int* satisfied) {
   return guarded([&]() { *satisfied = checked(opt)->stop_criterion_satisfied() ? 1 : 0; });
+}
+
+int fresco_opt_graph_stats(fresco_opt* opt, int64_t* captures, int64_t* replays) {
+  return guarded([&]() {
+    if (!captures || !replays) throw fresco_hip::InvalidArgument("null pointer");
+    checked(opt)->graph_stats(captures, replays);
+  });
 }
 
 int fresco_opt_get_state(fresco_opt* opt, double* f, double* rms, int* niter, int* nfev, int* maxiter, double* H0) {

@@ -266,47 +266,17 @@ class NUwU final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
-    last_x_ = x;
-    {
-      ProfScope p(ST_SORT, s);
-      pts_.set(x, map_g_, c_.get(), s);
-    }
-    spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
-    {
-      ProfScope p(ST_FFT_FWD, s);
-      r2c_.exec(grid_.get(), S_.get(), s);
-    }
-    {
-      ProfScope p(ST_ZERO, s);
-      for (int a = 0; a < nX_; ++a) fill_zero(X_[a].get(), nspec_, s);
-    }
-    {
-      ProfScope p(ST_LOSS, s);
-      if (plain_)
-        launch_loss<2>(s);
-      else
-        launch_loss<1>(s);
-      reduce_partials(partial_.get(), nblk_, e_dev_.get(), s);
-    }
-    if (plain_) {
-      for (int a = 0; a < d_; ++a) {
-        {
-          ProfScope p(ST_FFT_INV, s);
-          c2r_.exec(X_[a].get(), grid_.get(), s);
-        }
-        ProfScope p(ST_INTERP, s);
-        interpolate(pts_, grid_.get(), false, nullptr, grad, d_, a, s);
-      }
-    } else {
-      {
-        ProfScope p(ST_FFT_INV, s);
-        c2r_.exec(X_[0].get(), grid_.get(), s);
-      }
-      ProfScope p(ST_INTERP, s);
-      interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
-    }
+    enqueue_eg(x, grad, e_dev_.get(), s);
     return host_.fetch(e_dev_.get(), s);
   }
+
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  // the spreader's first call may time its candidates (host waits)
+  bool capture_ready() const override { return spreader_.tuned; }
+  uint64_t graph_epoch() const override { return epoch_; }
 
   void get_rho(double* rho, hipStream_t s) override {
     if (band_ < kp_) {
@@ -383,6 +353,48 @@ class NUwU final : public Potential {
   }
 
  private:
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
+    last_x_ = x;
+    {
+      ProfScope p(ST_SORT, s);
+      pts_.set(x, map_g_, c_.get(), s);
+    }
+    spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
+    {
+      ProfScope p(ST_FFT_FWD, s);
+      r2c_.exec(grid_.get(), S_.get(), s);
+    }
+    {
+      ProfScope p(ST_ZERO, s);
+      for (int a = 0; a < nX_; ++a) fill_zero(X_[a].get(), nspec_, s);
+    }
+    {
+      ProfScope p(ST_LOSS, s);
+      if (plain_)
+        launch_loss<2>(s);
+      else
+        launch_loss<1>(s);
+      reduce_partials(partial_.get(), nblk_, e_out, s);
+    }
+    if (plain_) {
+      for (int a = 0; a < d_; ++a) {
+        {
+          ProfScope p(ST_FFT_INV, s);
+          c2r_.exec(X_[a].get(), grid_.get(), s);
+        }
+        ProfScope p(ST_INTERP, s);
+        interpolate(pts_, grid_.get(), false, nullptr, grad, d_, a, s);
+      }
+    } else {
+      {
+        ProfScope p(ST_FFT_INV, s);
+        c2r_.exec(X_[0].get(), grid_.get(), s);
+      }
+      ProfScope p(ST_INTERP, s);
+      interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
+    }
+  }
+
   // largest |k_a| over modes whose pair {k, -k} carries weight; the FP64
   // plain path keeps the reference's full mode grid
   int active_band(const double* V) const {
@@ -403,6 +415,7 @@ class NUwU final : public Potential {
   // fine grid sized for the modes |k_a| <= band (sigma (2 band + 1) cells, the
   // NUFFT's accuracy for those modes is that of the full grid)
   void setup_grid(int band) {
+    ++epoch_;
     band_ = band;
     const int nmb = 2 * band + 1;
     int nf[3] = {1, 1, 1};
@@ -439,6 +452,7 @@ class NUwU final : public Potential {
   }
 
   void build_pairs(const double* Sk, const double* V) {
+    ++epoch_;
     std::vector<int> h, hn, ik, imk, kv;
     std::vector<double> vk, vm, sk, sm, ip;
     const int n = nm_;
@@ -538,6 +552,7 @@ class NUwU final : public Potential {
   DeviceBuffer<unsigned char> active_;               // mode belongs to an active pair
   DeviceBuffer<unsigned int> count_;
   HostScalar host_;
+  uint64_t epoch_ = 0;  // bumped whenever buffers or launch shapes change
   CoordMap map_e_, map_g_;
   double dscale_[3] = {0, 0, 0};
   double sigma_ = 2.0, eps_ = 1e-6, K_ = 0.0;

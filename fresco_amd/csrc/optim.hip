@@ -162,6 +162,7 @@ void VecOps::dots_dev(int nd, const double* a0, const double* b0, const double* 
 void VecOps::fetch(int cnt, double* dst, hipStream_t s) {
   FRESCO_HIP_CHECK(hipMemcpyAsync(host, scal.get(), cnt * sizeof(double), hipMemcpyDeviceToHost, s));
   FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+  if (prof_enabled()) prof_collect();
   for (int i = 0; i < cnt; ++i) dst[i] = host[i];
 }
 
@@ -218,6 +219,7 @@ void GradientOptimizer::compute_func_gradient(double* x, double& f, double* g) {
 void GradientOptimizer::initialize_func_gradient() {
   compute_func_gradient(x_.get(), f_, g_.get());
   double gg = ops_.dot(g_.get(), g_.get(), stream_);
+  gg_ = gg;
   rms_ = std::sqrt(gg / (double)n_);
   func_initialized_ = true;
 }
@@ -227,6 +229,7 @@ void GradientOptimizer::set_func_gradient(double f, const double* grad_host) {
   f_ = f;
   g_.upload(grad_host, n_, stream_);
   double gg = ops_.dot(g_.get(), g_.get(), stream_);
+  gg_ = gg;
   rms_ = std::sqrt(gg / (double)n_);
   func_initialized_ = true;
 }
@@ -258,10 +261,241 @@ void GradientOptimizer::get_g(double* host) {
 
 // ---------------------------------------------------------------------------
 // LBFGS (lbfgs.cpp)
+//
+// One iteration is a fixed sequence of kernels whose scalars (H0, rho, the
+// memory count k, the step factor, the energies) live in dsc_ on the device:
+//   begin       xold = x, gold = g, step = g (or the k = 0 first step)
+//   2 x M slots the two-loop recursion; slot kernels of memory entries not
+//               yet filled (k < M) return at once
+//   line        uphill test, step norm and factor of the first trial
+//   trial       x = xold + factor step, then the potential's evaluation
+//   memory      s, y of slot k % M and the sums ys, yy, g.g; committed (rho,
+//               H0, f, k += 1) only when the trial meets the energy-rise
+//               criterion
+// The same sequence runs launch by launch or, once the potential can be
+// recorded, as one replayed HIP graph; either way the host reads the scalars
+// back once.  A rejected first trial continues the backtracking from the host
+// (lbfgs.cpp:175-215) and then commits the memory unconditionally.
 // ---------------------------------------------------------------------------
 
+namespace {
+
+enum : int {
+  LS_F = 0,     // energy at the accepted x
+  LS_FNEW,      // energy of the last trial
+  LS_H0,        // lbfgs.cpp:100
+  LS_FACTOR,    // signed factor of the first trial (negative: step inverted)
+  LS_STEPNORM,  // |step|
+  LS_DOT,       // step . g before the inversion
+  LS_GG,        // g . g at the current x
+  LS_YS,        // raw y . s and y . y of the last memory update
+  LS_YY,
+  LS_K,           // memory updates so far
+  LS_ACCEPT,      // 1 when the last committed trial was accepted
+  LS_COEF,        // coefficient of the current two-loop slot
+  LS_MAXSTEP,     // parameters
+  LS_MAXRISE,
+  LS_REL,
+  LS_RHO = 16  // rho[M]
+};
+
+// memory slot i (0 <= i < M) of two-loop slot jj in loop 0 (backwards) or 1
+// (forwards), -1 when that entry is not filled yet (lbfgs.cpp:128-160)
+__device__ __forceinline__ int lb_slot(const double* dsc, int M, int loop, int jj) {
+  const int k = (int)dsc[LS_K];
+  const int cnt = k < M ? k : M;
+  if (jj >= cnt) return -1;
+  const int j = loop == 0 ? k - 1 - jj : (k - cnt) + jj;
+  return j % M;
+}
+
+__global__ void k_lb_set(double* __restrict__ dsc, int i0, double v0, int i1, double v1, int i2, double v2, int i3,
+                         double v3) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (i0 >= 0) dsc[i0] = v0;
+    if (i1 >= 0) dsc[i1] = v1;
+    if (i2 >= 0) dsc[i2] = v2;
+    if (i3 >= 0) dsc[i3] = v3;
+  }
+}
+
+// lbfgs.cpp:110-127 and the copies of lbfgs.cpp:40-41
+__global__ void k_lb_begin(const double* __restrict__ x, const double* __restrict__ g, double* __restrict__ xold,
+                           double* __restrict__ gold, double* __restrict__ step, const double* __restrict__ dsc,
+                           int64_t n) {
+  const bool first = dsc[LS_K] == 0.0;
+  double pref = 1.0;
+  if (first) {
+    double gn = sqrt(dsc[LS_GG]);
+    if (gn > 1.) gn = 1. / gn;
+    pref = -gn * dsc[LS_H0];
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double gi = g[i];
+    xold[i] = x[i];
+    gold[i] = gi;
+    step[i] = first ? pref * gi : gi;
+  }
+}
+
+// partial sums of v_i . step for the memory entry of a two-loop slot
+__global__ void __launch_bounds__(RB)
+    k_lb_dot(const double* __restrict__ vecs, const double* __restrict__ step, int64_t n,
+             const double* __restrict__ dsc, int M, int loop, int jj, double* __restrict__ part) {
+  const int i = lb_slot(dsc, M, loop, jj);
+  if (i < 0) return;
+  __shared__ double sh[RB];
+  const double* v = vecs + (int64_t)i * n;
+  double acc = 0;
+  for (int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x; t < n; t += (int64_t)gridDim.x * RB) acc += v[t] * step[t];
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// loop 0: alpha_i = rho_i (s_i . step)            (lbfgs.cpp:133-138)
+// loop 1: coef = alpha_i - beta, beta = -rho_i (y_i . step)  (lbfgs.cpp:149-155)
+__global__ void __launch_bounds__(RB)
+    k_lb_coef(const double* __restrict__ part, int nb, double* __restrict__ dsc, double* __restrict__ alpha, int M,
+              int loop, int jj) {
+  const int i = lb_slot(dsc, M, loop, jj);
+  if (i < 0) return;
+  __shared__ double sh[RB];
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) s += part[b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const double r = dsc[LS_RHO + i] * s;
+    if (loop == 0) {
+      alpha[i] = r;
+      dsc[LS_COEF] = r;
+    } else {
+      dsc[LS_COEF] = alpha[i] + r;
+    }
+  }
+}
+
+// step -= coef v_i   (lbfgs.cpp:139-142, 157-160)
+__global__ void k_lb_axpy(double* __restrict__ step, const double* __restrict__ vecs, int64_t n,
+                          const double* __restrict__ dsc, int M, int loop, int jj) {
+  const int i = lb_slot(dsc, M, loop, jj);
+  if (i < 0) return;
+  const double c = dsc[LS_COEF];
+  const double* v = vecs + (int64_t)i * n;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    step[t] -= c * v[t];
+}
+
+// step *= -H0 once the memory holds an entry (lbfgs.cpp:144-147)
+__global__ void k_lb_scale(double* __restrict__ step, const double* __restrict__ dsc, int64_t n) {
+  if (dsc[LS_K] == 0.0) return;
+  const double c = -dsc[LS_H0];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    step[t] *= c;
+}
+
+// uphill inversion and the maxstep cap (lbfgs.cpp:166-186) from the sums
+// (step . g, step . step)
+__global__ void __launch_bounds__(RB) k_lb_line(const double* __restrict__ part, int nb, double* __restrict__ dsc) {
+  __shared__ double sh[RB];
+  double s0 = 0, s1 = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) {
+    s0 += part[b];
+    s1 += part[RMAXB + b];
+  }
+  s0 = block_sum(s0, sh);
+  s1 = block_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    const double sn = sqrt(s1);
+    double factor = 1.;
+    if (factor * sn > dsc[LS_MAXSTEP]) factor = dsc[LS_MAXSTEP] / sn;
+    dsc[LS_DOT] = s0;
+    dsc[LS_STEPNORM] = sn;
+    dsc[LS_FACTOR] = s0 > 0. ? -factor : factor;
+  }
+}
+
+// x = xold + factor step (lbfgs.cpp:193)
+__global__ void k_lb_trial(double* __restrict__ x, const double* __restrict__ xold, const double* __restrict__ step,
+                           const double* __restrict__ dsc, int64_t n) {
+  const double f = dsc[LS_FACTOR];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    x[t] = xold[t] + f * step[t];
+}
+
+// y_k = g - gold, s_k = x - xold in slot k % M; sums ys, yy, g.g
+// (lbfgs.cpp:66-84)
+__global__ void __launch_bounds__(RB)
+    k_lb_memory(const double* __restrict__ x, const double* __restrict__ xold, const double* __restrict__ g,
+                const double* __restrict__ gold, double* __restrict__ sbase, double* __restrict__ ybase, int64_t n,
+                const double* __restrict__ dsc, int M, double* __restrict__ part) {
+  __shared__ double sh[RB];
+  const int kl = (int)dsc[LS_K] % M;
+  double* s = sbase + (int64_t)kl * n;
+  double* y = ybase + (int64_t)kl * n;
+  double ys = 0, yy = 0, gg = 0;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    const double gi = g[i];
+    const double yv = gi - gold[i];
+    const double sv = x[i] - xold[i];
+    y[i] = yv;
+    s[i] = sv;
+    ys += yv * sv;
+    yy += yv * yv;
+    gg += gi * gi;
+  }
+  ys = block_sum(ys, sh);
+  yy = block_sum(yy, sh);
+  gg = block_sum(gg, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = ys;
+    part[RMAXB + blockIdx.x] = yy;
+    part[2 * RMAXB + blockIdx.x] = gg;
+  }
+}
+
+// energy-rise test of the trial (lbfgs.cpp:195-204); on acceptance (or
+// force) rho, H0, f and k as in lbfgs.cpp:86-104
+__global__ void __launch_bounds__(RB)
+    k_lb_commit(const double* __restrict__ part, int nb, double* __restrict__ dsc, int M, int force) {
+  __shared__ double sh[RB];
+  double ys = 0, yy = 0, gg = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) {
+    ys += part[b];
+    yy += part[RMAXB + b];
+    gg += part[2 * RMAXB + b];
+  }
+  ys = block_sum(ys, sh);
+  yy = block_sum(yy, sh);
+  gg = block_sum(gg, sh);
+  if (threadIdx.x == 0) {
+    const double f = dsc[LS_F], fnew = dsc[LS_FNEW];
+    double df = fnew - f;
+    if (dsc[LS_REL] != 0.0) {
+      double absf = 1e-100;
+      if (f != 0) absf = fabs(f);
+      df /= absf;
+    }
+    const bool accept = force != 0 || df < dsc[LS_MAXRISE];
+    dsc[LS_GG] = gg;
+    dsc[LS_YS] = ys;
+    dsc[LS_YY] = yy;
+    dsc[LS_ACCEPT] = accept ? 1.0 : 0.0;
+    if (accept) {
+      const int k = (int)dsc[LS_K];
+      const double ys1 = ys == 0. ? 1. : ys;
+      const double yy1 = yy == 0. ? 1. : yy;
+      dsc[LS_RHO + k % M] = 1. / ys1;
+      dsc[LS_H0] = ys1 / yy1;
+      dsc[LS_F] = fnew;
+      dsc[LS_K] = (double)(k + 1);
+    }
+  }
+}
+
+}  // namespace
+
 LBFGS::LBFGS(std::shared_ptr<Potential> pot, const double* x0, int64_t n, double tol, int M)
-    : GradientOptimizer(std::move(pot), x0, n, tol), M_(M), rho_(M, 0.0) {
+    : GradientOptimizer(std::move(pot), x0, n, tol), M_(M) {
   if (M < 1) throw InvalidArgument("LBFGS memory M must be >= 1");
   inv_sqrt_size_ = 1.0 / std::sqrt((double)n);
   xold_.allocate(n);
@@ -270,117 +504,216 @@ LBFGS::LBFGS(std::shared_ptr<Potential> pot, const double* x0, int64_t n, double
   s_.allocate((size_t)n * M);
   y_.allocate((size_t)n * M);
   alpha_.allocate(M);
-  coef_.allocate(1);
+  dsc_.allocate(LS_RHO + (size_t)M);
+  FRESCO_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hsc_), LS_RHO * sizeof(double)));
+  // rho_ starts at 0 (lbfgs.cpp:21) and is only read for filled entries
+  std::vector<double> init(LS_RHO + (size_t)M, 0.0);
+  init[LS_H0] = H0_;
+  init[LS_MAXSTEP] = maxstep_;
+  init[LS_MAXRISE] = max_f_rise_;
+  init[LS_REL] = use_relative_f_ ? 1.0 : 0.0;
+  dsc_.upload(init.data(), init.size(), stream_);
+}
+
+LBFGS::~LBFGS() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (cap_) (void)hipStreamDestroy(cap_);
+  if (hsc_) (void)hipHostFree(hsc_);
+}
+
+void LBFGS::set_H0(double H0) {
+  if (iter_number_ > 0) std::cout << "warning: setting H0 after the first iteration.\n";
+  H0_ = H0;
+  set_scalar(LS_H0, H0);
+}
+
+void LBFGS::set_maxstep(double v) {
+  maxstep_ = v;
+  set_scalar(LS_MAXSTEP, v);
+}
+
+void LBFGS::set_max_f_rise(double v) {
+  max_f_rise_ = v;
+  set_scalar(LS_MAXRISE, v);
+}
+
+void LBFGS::set_use_relative_f(int v) {
+  use_relative_f_ = v != 0;
+  set_scalar(LS_REL, use_relative_f_ ? 1.0 : 0.0);
+}
+
+void LBFGS::set_scalar(int idx, double v) {
+  hipLaunchKernelGGL(k_lb_set, dim3(1), dim3(64), 0, stream_, dsc_.get(), idx, v, -1, 0.0, -1, 0.0, -1, 0.0);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+void LBFGS::initialize_func_gradient() {
+  GradientOptimizer::initialize_func_gradient();
+  hipLaunchKernelGGL(k_lb_set, dim3(1), dim3(64), 0, stream_, dsc_.get(), (int)LS_F, f_, (int)LS_GG, gg_, -1, 0.0, -1,
+                     0.0);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+void LBFGS::set_func_gradient(double f, const double* grad_host) {
+  GradientOptimizer::set_func_gradient(f, grad_host);
+  hipLaunchKernelGGL(k_lb_set, dim3(1), dim3(64), 0, stream_, dsc_.get(), (int)LS_F, f_, (int)LS_GG, gg_, -1, 0.0, -1,
+                     0.0);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+void LBFGS::read_scalars() {
+  FRESCO_HIP_CHECK(hipMemcpyAsync(hsc_, dsc_.get(), LS_RHO * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  FRESCO_HIP_CHECK(hipStreamSynchronize(stream_));
+  if (prof_enabled()) prof_collect();
+}
+
+// trial x = xold + factor step and its evaluation; the energy lands in
+// dsc_[LS_FNEW].  factor < 0 in `host_factor` means: take LS_FACTOR from the
+// device.  Returns false when the potential has no enqueue-only evaluation
+// and the evaluation ran synchronously instead.
+void LBFGS::enqueue_trial(bool dev_factor, double host_factor, hipStream_t s) {
+  const unsigned eb = eblocks(n_);
+  if (dev_factor)
+    hipLaunchKernelGGL(k_lb_trial, dim3(eb), dim3(256), 0, s, x_.get(), xold_.get(), step_.get(), dsc_.get(), n_);
+  else
+    hipLaunchKernelGGL(k_axpbz, dim3(eb), dim3(256), 0, s, x_.get(), 1.0, xold_.get(), host_factor, step_.get(), n_);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  nfev_ += 1;
+  if (!pot_->energy_gradient_async(x_.get(), g_.get(), dsc_.get() + LS_FNEW, s)) {
+    const double f = pot_->energy_gradient(x_.get(), g_.get(), s);
+    hipLaunchKernelGGL(k_lb_set, dim3(1), dim3(64), 0, s, dsc_.get(), (int)LS_FNEW, f, -1, 0.0, -1, 0.0, -1, 0.0);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+}
+
+void LBFGS::enqueue_commit(int force, hipStream_t s) {
+  const unsigned nb = nblocks(n_);
+  hipLaunchKernelGGL(k_lb_memory, dim3(nb), dim3(RB), 0, s, x_.get(), xold_.get(), g_.get(), gold_.get(), s_.get(),
+                     y_.get(), n_, dsc_.get(), M_, ops_.part.get());
+  hipLaunchKernelGGL(k_lb_commit, dim3(1), dim3(RB), 0, s, ops_.part.get(), (int)nb, dsc_.get(), M_, force);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+// direction, first trial and the conditional memory update (lbfgs.cpp:31-44)
+void LBFGS::enqueue_iteration(hipStream_t s) {
+  const unsigned eb = eblocks(n_);
+  const unsigned nb = nblocks(n_);
+  double* part = ops_.part.get();
+  double* dsc = dsc_.get();
+  hipLaunchKernelGGL(k_lb_begin, dim3(eb), dim3(256), 0, s, x_.get(), g_.get(), xold_.get(), gold_.get(), step_.get(),
+                     dsc, n_);
+  for (int jj = 0; jj < M_; ++jj) {
+    hipLaunchKernelGGL(k_lb_dot, dim3(nb), dim3(RB), 0, s, s_.get(), step_.get(), n_, dsc, M_, 0, jj, part);
+    hipLaunchKernelGGL(k_lb_coef, dim3(1), dim3(RB), 0, s, part, (int)nb, dsc, alpha_.get(), M_, 0, jj);
+    hipLaunchKernelGGL(k_lb_axpy, dim3(eb), dim3(256), 0, s, step_.get(), y_.get(), n_, dsc, M_, 0, jj);
+  }
+  hipLaunchKernelGGL(k_lb_scale, dim3(eb), dim3(256), 0, s, step_.get(), dsc, n_);
+  for (int jj = 0; jj < M_; ++jj) {
+    hipLaunchKernelGGL(k_lb_dot, dim3(nb), dim3(RB), 0, s, y_.get(), step_.get(), n_, dsc, M_, 1, jj, part);
+    hipLaunchKernelGGL(k_lb_coef, dim3(1), dim3(RB), 0, s, part, (int)nb, dsc, alpha_.get(), M_, 1, jj);
+    hipLaunchKernelGGL(k_lb_axpy, dim3(eb), dim3(256), 0, s, step_.get(), s_.get(), n_, dsc, M_, 1, jj);
+  }
+  hipLaunchKernelGGL(k_dots, dim3(nb), dim3(RB), 0, s, step_.get(), g_.get(), step_.get(), step_.get(), nullptr,
+                     nullptr, n_, 2, part);
+  hipLaunchKernelGGL(k_lb_line, dim3(1), dim3(RB), 0, s, part, (int)nb, dsc);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  enqueue_trial(true, 0.0, s);
+  enqueue_commit(0, s);
+}
+
+bool LBFGS::run_graph() {
+  const uint64_t ep = pot_->graph_epoch();
+  if (!gexec_ || ep != gepoch_) {
+    if (gexec_) {
+      FRESCO_HIP_CHECK(hipGraphExecDestroy(gexec_));
+      gexec_ = nullptr;
+    }
+    if (!cap_) FRESCO_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+    // the captured evaluation must not count as one
+    const int nfev = nfev_;
+    hipGraph_t g = nullptr;
+    FRESCO_HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_iteration(cap_);
+    } catch (...) {
+      (void)hipStreamEndCapture(cap_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      nfev_ = nfev;
+      throw;
+    }
+    nfev_ = nfev;
+    FRESCO_HIP_CHECK(hipStreamEndCapture(cap_, &g));
+    const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FRESCO_HIP_CHECK(e);
+    gepoch_ = ep;
+    ++captures_;
+  }
+  FRESCO_HIP_CHECK(hipGraphLaunch(gexec_, stream_));
+  nfev_ += 1;
+  ++replays_;
+  return true;
 }
 
 void LBFGS::one_iteration() {
   if (!func_initialized_) initialize_func_gradient();
-  ops_.copy(xold_.get(), x_.get(), stream_);
-  ops_.copy(gold_.get(), g_.get(), stream_);
-  compute_lbfgs_step();
-  double stepnorm = backtracking_linesearch();
-  update_memory();
+  // lbfgs.cpp:31-62
+  const bool graph = use_graph_ && pot_->capture_ready() && !prof_enabled();
+  if (graph)
+    run_graph();
+  else
+    enqueue_iteration(stream_);
+  read_scalars();
+  const double f0 = hsc_[LS_F];
+  if (hsc_[LS_DOT] > 0. && verbosity_ > 1) std::cout << "warning: step direction was uphill.  inverting" << std::endl;
+  const double stepnorm = hsc_[LS_STEPNORM];
+  double factor = std::fabs(hsc_[LS_FACTOR]);
+  double fnew = hsc_[LS_FNEW];
+  if (hsc_[LS_ACCEPT] == 0.0) {
+    // backtracking continues from the second trial (lbfgs.cpp:188-215)
+    const double sign = hsc_[LS_FACTOR] < 0 ? -1.0 : 1.0;
+    const int nred_max = 10;
+    int nred = 0;
+    for (;;) {
+      double df = fnew - f0;
+      if (use_relative_f_) {
+        double absf = 1e-100;
+        if (f0 != 0) absf = std::abs(f0);
+        df /= absf;
+      }
+      if (df < max_f_rise_) break;
+      factor *= 0.5;
+      if (verbosity_ > 2)
+        std::cout << "energy increased by " << df << " to " << fnew << " from " << f0 << " reducing step norm to "
+                  << factor * stepnorm << " H0 " << hsc_[LS_H0] << std::endl;
+      if (++nred >= nred_max) break;
+      enqueue_trial(false, sign * factor, stream_);
+      read_scalars();
+      fnew = hsc_[LS_FNEW];
+    }
+    if (nred >= nred_max && verbosity_ > 0) std::cout << "warning: the line search backtracked too many times" << std::endl;
+    enqueue_commit(1, stream_);
+    read_scalars();
+  }
+  if (verbosity_ > 0 && hsc_[LS_YS] == 0.) std::cout << "warning: resetting YS to 1." << std::endl;
+  if (verbosity_ > 0 && hsc_[LS_YY] == 0.) std::cout << "warning: resetting YY to 1." << std::endl;
+  f_ = hsc_[LS_F];
+  gg_ = hsc_[LS_GG];
+  rms_ = std::sqrt(gg_) * inv_sqrt_size_;
+  H0_ = hsc_[LS_H0];
+  k_ = (int)hsc_[LS_K];
   if (iprint_ > 0 && iter_number_ % iprint_ == 0) {
     std::cout << std::setprecision(std::numeric_limits<double>::max_digits10) << "lbgs: " << iter_number_ << " E "
-              << f_ << " rms " << rms_ << " nfev " << nfev_ << " step norm " << stepnorm << std::endl;
+              << f_ << " rms " << rms_ << " nfev " << nfev_ << " step norm " << stepnorm * factor << std::endl;
   }
   iter_number_ += 1;
-}
-
-void LBFGS::update_memory() {
-  const int kl = k_ % M_;
-  double* s = s_.get() + (size_t)kl * n_;
-  double* y = y_.get() + (size_t)kl * n_;
-  unsigned nb = nblocks(n_);
-  hipLaunchKernelGGL(k_lbfgs_memory, dim3(nb), dim3(RB), 0, stream_, x_.get(), xold_.get(), g_.get(), gold_.get(), s,
-                     y, n_, ops_.part.get());
-  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(RB), 0, stream_, ops_.part.get(), (int)nb, 2, 1.0, nullptr,
-                     ops_.scal.get());
-  FRESCO_HIP_CHECK(hipGetLastError());
-  double r[2];
-  ops_.fetch(2, r, stream_);
-  double ys = r[0], yy = r[1];
-  if (ys == 0.) {
-    if (verbosity_ > 0) std::cout << "warning: resetting YS to 1." << std::endl;
-    ys = 1.;
-  }
-  rho_[kl] = 1. / ys;
-  if (yy == 0.) {
-    if (verbosity_ > 0) std::cout << "warning: resetting YY to 1." << std::endl;
-    yy = 1.;
-  }
-  H0_ = ys / yy;
-  k_ += 1;
-}
-
-void LBFGS::compute_lbfgs_step() {
-  if (k_ == 0) {
-    double gnorm = std::sqrt(ops_.dot(g_.get(), g_.get(), stream_));
-    if (gnorm > 1.) gnorm = 1. / gnorm;
-    ops_.axpbz(step_.get(), -gnorm * H0_, g_.get(), 0.0, nullptr, stream_);
-    return;
-  }
-  ops_.copy(step_.get(), g_.get(), stream_);
-  const int jmin = std::max(0, k_ - M_);
-  const int jmax = k_;
-  for (int j = jmax - 1; j >= jmin; --j) {
-    const int i = j % M_;
-    const double* s = s_.get() + (size_t)i * n_;
-    const double* y = y_.get() + (size_t)i * n_;
-    // alpha_i = rho_i s_i.step ; step -= alpha_i y_i
-    ops_.dots_dev(1, s, step_.get(), nullptr, nullptr, nullptr, nullptr, rho_[i], nullptr, alpha_.get() + i,
-                  stream_);
-    ops_.axpy_dev(step_.get(), y, alpha_.get() + i, -1.0, stream_);
-  }
-  ops_.axpbz(step_.get(), -H0_, step_.get(), 0.0, nullptr, stream_);
-  for (int j = jmin; j < jmax; ++j) {
-    const int i = j % M_;
-    const double* s = s_.get() + (size_t)i * n_;
-    const double* y = y_.get() + (size_t)i * n_;
-    // beta = -rho_i y_i.step ; step -= s_i (alpha_i - beta)
-    ops_.dots_dev(1, y, step_.get(), nullptr, nullptr, nullptr, nullptr, rho_[i], alpha_.get() + i, coef_.get(),
-                  stream_);
-    ops_.axpy_dev(step_.get(), s, coef_.get(), -1.0, stream_);
-  }
-}
-
-double LBFGS::backtracking_linesearch() {
-  double dotprod = ops_.dot(step_.get(), g_.get(), stream_);
-  if (dotprod > 0.) {
-    if (verbosity_ > 1) std::cout << "warning: step direction was uphill.  inverting" << std::endl;
-    ops_.axpbz(step_.get(), -1.0, step_.get(), 0.0, nullptr, stream_);
-  }
-  double factor = 1.;
-  double stepnorm = std::sqrt(ops_.dot(step_.get(), step_.get(), stream_));
-  if (factor * stepnorm > maxstep_) factor = maxstep_ / stepnorm;
-  double fnew = f_;
-  int nred;
-  const int nred_max = 10;
-  for (nred = 0; nred < nred_max; ++nred) {
-    ops_.axpbz(x_.get(), 1.0, xold_.get(), factor, step_.get(), stream_);
-    compute_func_gradient(x_.get(), fnew, g_.get());
-    double df = fnew - f_;
-    if (use_relative_f_) {
-      double absf = 1e-100;
-      if (f_ != 0) absf = std::abs(f_);
-      df /= absf;
-    }
-    if (df < max_f_rise_) break;
-    factor *= 0.5;
-    if (verbosity_ > 2)
-      std::cout << "energy increased by " << df << " to " << fnew << " from " << f_ << " reducing step norm to "
-                << factor * stepnorm << " H0 " << H0_ << std::endl;
-  }
-  if (nred >= nred_max && verbosity_ > 0) std::cout << "warning: the line search backtracked too many times" << std::endl;
-  f_ = fnew;
-  rms_ = std::sqrt(ops_.dot(g_.get(), g_.get(), stream_)) * inv_sqrt_size_;
-  return stepnorm * factor;
 }
 
 void LBFGS::reset(const double* x0_host) {
   k_ = 0;
   iter_number_ = 0;
   nfev_ = 0;
+  set_scalar(LS_K, 0.0);
   x_.upload(x0_host, n_, stream_);
   initialize_func_gradient();
 }

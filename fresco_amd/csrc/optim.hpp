@@ -48,7 +48,7 @@ class GradientOptimizer {
   bool stop_criterion_satisfied();
 
   void set_tol(double v) { tol_ = v; }
-  void set_maxstep(double v) { maxstep_ = v; }
+  virtual void set_maxstep(double v) { maxstep_ = v; }
   void set_max_iter(int v) { maxiter_ = v; }
   void set_iprint(int v) { iprint_ = v; }
   void set_verbosity(int v) { verbosity_ = v; }
@@ -66,6 +66,12 @@ class GradientOptimizer {
   void get_g(double* host);
   double* x_device() { return x_.get(); }
   double* g_device() { return g_.get(); }
+  // HIP graph replay of whole iterations (where the optimizer supports it)
+  virtual void set_use_graph(bool on) { (void)on; }
+  virtual void graph_stats(int64_t* captures, int64_t* replays) const {
+    *captures = 0;
+    *replays = 0;
+  }
 
  protected:
   void compute_func_gradient(double* x, double& f, double* g);
@@ -82,38 +88,57 @@ class GradientOptimizer {
   DeviceBuffer<double> x_, g_;
   double f_ = 0.;
   double rms_ = 1e10;
+  double gg_ = 0.;  // g . g at x
   bool func_initialized_ = false;
   hipStream_t stream_ = nullptr;
   VecOps ops_;
 };
 
-// lbfgs.cpp:9-257
+// lbfgs.cpp:9-257.  Scalars (H0, rho, k, step factor, energies) live on the
+// device so that one iteration is a fixed kernel sequence, replayed as a HIP
+// graph when the potential allows it (see optim.hip).
 class LBFGS final : public GradientOptimizer {
  public:
   LBFGS(std::shared_ptr<Potential> pot, const double* x0, int64_t n, double tol, int M);
+  ~LBFGS() override;
+  LBFGS(const LBFGS&) = delete;
+  LBFGS& operator=(const LBFGS&) = delete;
   void one_iteration() override;
   void reset(const double* x0_host) override;
-  void set_H0(double H0) {
-    if (iter_number_ > 0) std::cout << "warning: setting H0 after the first iteration.\n";
-    H0_ = H0;
-  }
-  void set_max_f_rise(double v) { max_f_rise_ = v; }
-  void set_use_relative_f(int v) { use_relative_f_ = v != 0; }
+  void initialize_func_gradient() override;
+  void set_func_gradient(double f, const double* grad_host) override;
+  void set_H0(double H0);
+  void set_maxstep(double v) override;
+  void set_max_f_rise(double v);
+  void set_use_relative_f(int v);
   double get_H0() const override { return H0_; }
+  void set_use_graph(bool on) override { use_graph_ = on; }
+  void graph_stats(int64_t* captures, int64_t* replays) const override {
+    *captures = captures_;
+    *replays = replays_;
+  }
 
  private:
-  void update_memory();
-  void compute_lbfgs_step();
-  double backtracking_linesearch();
+  void set_scalar(int idx, double v);
+  void read_scalars();
+  void enqueue_iteration(hipStream_t s);
+  void enqueue_trial(bool dev_factor, double host_factor, hipStream_t s);
+  void enqueue_commit(int force, hipStream_t s);
+  bool run_graph();
 
   int M_;
   double max_f_rise_ = 1e-4;
   bool use_relative_f_ = false;
-  std::vector<double> rho_;
   double H0_ = 0.1;
   int k_ = 0;
   double inv_sqrt_size_ = 1.0;
-  DeviceBuffer<double> xold_, gold_, step_, s_, y_, alpha_, coef_;
+  DeviceBuffer<double> xold_, gold_, step_, s_, y_, alpha_, dsc_;
+  double* hsc_ = nullptr;  // pinned mirror of the device scalars
+  bool use_graph_ = true;
+  hipStream_t cap_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  uint64_t gepoch_ = 0;
+  int64_t captures_ = 0, replays_ = 0;
 };
 
 // modified_fire.hpp:44-230

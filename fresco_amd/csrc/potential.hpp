@@ -22,6 +22,21 @@ class Potential {
   virtual double energy(const double* x, hipStream_t s) = 0;
   // base_potential.hpp:38-43; grad (device, ndof) is overwritten
   virtual double energy_gradient(const double* x, double* grad, hipStream_t s) = 0;
+  // energy_gradient without the wait: enqueues the evaluation on s and
+  // leaves the energy in *e_dev (device).  Returns false (enqueuing nothing)
+  // when the potential has no such path.  What it enqueues is kernels only
+  // (no memory fills, host waits or allocations), so it may be recorded
+  // into a HIP graph once capture_ready() holds; graph_epoch() changes
+  // whenever later calls would enqueue other work or other buffers.
+  virtual bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) {
+    (void)x;
+    (void)grad;
+    (void)e_dev;
+    (void)s;
+    return false;
+  }
+  virtual bool capture_ready() const { return false; }
+  virtual uint64_t graph_epoch() const { return 0; }
   // rho_k of the last evaluation, complex interleaved float64 (device)
   virtual void get_rho(double* rho, hipStream_t s) {
     (void)rho;

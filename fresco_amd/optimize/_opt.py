@@ -126,11 +126,16 @@ class GradientOptimizer:
 
 
 class LBFGS_CPP(GradientOptimizer):
-    """LBFGS with backtracking line search on the GPU (lbfgs.cpp)."""
+    """LBFGS with backtracking line search on the GPU (lbfgs.cpp).
+
+    use_graph (not in the reference): replay each iteration as one recorded
+    HIP graph when the potential can be recorded (NUwU, batched NUwU);
+    False launches the same kernels one by one, with identical results.
+    """
 
     def __init__(self, x0, potential, tol=1e-5, M=4, maxstep=0.1, maxErise=1e-4, H0=0.1, iprint=-1,
                  energy=None, gradient=None, nsteps=10000, verbosity=0, events=None, logger=None,
-                 rel_energy=False):
+                 rel_energy=False, use_graph=True):
         x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).ravel())
         self._n = len(x)
         self.pot = as_cpp_potential(potential, verbose=verbosity > 0, ndof=self._n)
@@ -148,6 +153,7 @@ class LBFGS_CPP(GradientOptimizer):
         self._set(_lib.OPT_IPRINT, iprint)
         if rel_energy:
             self._set(_lib.OPT_USE_RELATIVE_F, 1)
+        self._set(_lib.OPT_USE_GRAPH, 1 if use_graph else 0)
         if energy is not None and gradient is not None:
             g = np.ascontiguousarray(np.asarray(gradient, dtype=np.float64).ravel())
             _lib.check(_lib.load().fresco_opt_set_func_gradient(self._h, float(energy), _lib.dptr(g), len(g)))
@@ -155,6 +161,12 @@ class LBFGS_CPP(GradientOptimizer):
 
     def set_H0(self, H0):
         self._set(_lib.OPT_H0, float(H0))
+
+    def graph_stats(self):
+        """(graphs recorded, iterations replayed from a graph)."""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.load().fresco_opt_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return int(c.value), int(r.value)
 
     def get_result(self):
         res = super().get_result()

@@ -51,6 +51,10 @@ typedef int (*fresco_eg_callback)(const double* x, double* grad, double* energy,
 #define FRESCO_OPT_H0 5             /* lbfgs.hpp:70-76 set_H0 */
 #define FRESCO_OPT_MAX_F_RISE 6     /* lbfgs.hpp:77 set_max_f_rise */
 #define FRESCO_OPT_USE_RELATIVE_F 7 /* lbfgs.hpp:79-82 set_use_relative_f */
+/* No counterpart in the reference: 1 (default) lets LBFGS replay each
+ * iteration as one recorded HIP graph when the potential supports it; 0
+ * launches the same kernels one by one.  Both give identical results. */
+#define FRESCO_OPT_USE_GRAPH 8
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* fresco_last_error(void);
@@ -297,6 +301,11 @@ int fresco_opt_set_func_gradient(fresco_opt* opt, double f, const double* grad, 
 
 /* stop_criterion_satisfied (optimizer.hpp:195-201). */
 int fresco_opt_stop_criterion(fresco_opt* opt, int* satisfied);
+
+/* No counterpart in the reference: how many times LBFGS recorded its
+ * iteration as a HIP graph and how many iterations replayed one (0, 0 for
+ * FIRE and for potentials that cannot be recorded). */
+int fresco_opt_graph_stats(fresco_opt* opt, int64_t* captures, int64_t* replays);
 
 /* get_f / get_rms / get_niter / get_nfev / get_maxiter / get_H0 (optimizer.hpp:181-189,
  * lbfgs.hpp:86); any pointer may be NULL. */

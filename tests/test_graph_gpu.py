@@ -1,0 +1,144 @@
+"""LBFGS iterations replayed from a recorded HIP graph against the same
+kernels launched one by one.
+
+Both optimizers drive ONE potential object and alternate: every replay is
+followed by eager work on that potential (an evaluation at an unrelated x and
+the launch-by-launch optimizer's own iteration, each ending in a host
+synchronisation), and after every iteration the two trajectories must agree
+bit for bit (coordinates, energy, H0, rms, evaluation count).  The recorded
+path is the product default; the launch-by-launch path is the reference
+structure of lbfgs.cpp:31-64 (its parity with the optimizer oracle is covered
+in test_optim_gpu.py)."""
+import numpy as np
+import pytest
+
+from oracle import sf_direct as D
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from fresco_amd.optimize import LBFGS_CPP
+    import fresco_amd.potentials as P
+
+    return LBFGS_CPP, P
+
+
+def stealthy(kp, d, kmax):
+    k = D.mode_grid(kp, d)
+    kk = np.linalg.norm(k, axis=1)
+    V = ((kk > 0) & (kk <= kmax)).astype(float)
+    return np.zeros_like(V), V
+
+
+def make_case(P, name):
+    rng = np.random.default_rng(7)
+    if name == "nuwu2d_fp64":
+        N, kp = 400, 12
+        Sk, V = stealthy(kp, 2, 6)
+        pot = P.NUwU(np.ones(N), kp, Sk, V, np.ones(2), 1e-12, precision="fp64")
+        return pot, rng.uniform(0, 1, 2 * N)
+    if name == "nuwu2d_fp32":
+        N, kp = 4096, 32
+        Sk, V = stealthy(kp, 2, 16)
+        pot = P.NUwU(np.ones(N), kp, Sk, V, np.ones(2), 1e-6, precision="fp32")
+        return pot, rng.uniform(0, 1, 2 * N)
+    if name == "nuwu3d_fp32":
+        N, kp = 3000, 10
+        Sk, V = stealthy(kp, 3, 6)
+        pot = P.NUwU(np.ones(N), kp, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        return pot, rng.uniform(0, 1, 3 * N)
+    if name == "batch2d_fp32":
+        pats, xs = [], []
+        for N, n in ((60, 9), (300, 17), (150, 13)):
+            Sk, V = stealthy(n // 2, 2, n // 3)
+            pats.append((np.ones(N), Sk, V))
+            xs.append(rng.uniform(0, 1, 2 * N))
+        return P.NUwUBatch(pats, np.ones(2), 1e-6, precision="fp32"), np.concatenate(xs)
+    raise ValueError(name)
+
+
+def same(ra, rb):
+    assert np.array_equal(ra.coords, rb.coords)
+    assert ra.energy == rb.energy and ra.rms == rb.rms
+    assert ra["H0"] == rb["H0"] and ra.nfev == rb.nfev and ra.nsteps == rb.nsteps
+
+
+@pytest.mark.parametrize("name", ["nuwu2d_fp64", "nuwu2d_fp32", "nuwu3d_fp32", "batch2d_fp32"])
+def test_graph_replays_match_launches(mods, name):
+    LBFGS_CPP, P = mods
+    pot, x0 = make_case(P, name)
+    print("case", name, "ndof", len(x0), flush=True)
+    ga = LBFGS_CPP(x0, pot, tol=0.0, M=4, use_graph=True)
+    eb = LBFGS_CPP(x0, pot, tol=0.0, M=4, use_graph=False)
+    xr = np.random.default_rng(1).uniform(0, 1, len(x0))
+    for it in range(30):
+        ra = ga.one_iteration()
+        pot.get_energy_gradient(xr)  # eager work between replays
+        rb = eb.one_iteration()
+        same(ra, rb)
+    assert ga.graph_stats() == (1, 30)
+    assert eb.graph_stats() == (0, 0)
+    print(name, "E", ra.energy, "rms", ra.rms, "nfev", ra.nfev, flush=True)
+
+
+def test_graph_backtracking_and_memory_wrap(mods):
+    """Demanding a 60% relative energy drop per step (a negative maxErise with
+    rel_energy) makes first trials fail: the host continues the backtracking
+    after the replay, sometimes through all 10 trials; M=3 wraps the memory
+    ring many times."""
+    LBFGS_CPP, P = mods
+    pot, x0 = make_case(P, "nuwu2d_fp64")
+    kw = dict(tol=0.0, M=3, maxstep=1.0, maxErise=-0.6, rel_energy=True)
+    ga = LBFGS_CPP(x0, pot, use_graph=True, **kw)
+    eb = LBFGS_CPP(x0, pot, use_graph=False, **kw)
+    for it in range(40):
+        ra = ga.one_iteration()
+        rb = eb.one_iteration()
+        same(ra, rb)
+    print("nfev", ra.nfev, "iterations", ra.nsteps, flush=True)
+    assert ra.nfev > ra.nsteps + 20  # backtracking happened (oracle: 140 / 40)
+    assert ga.graph_stats() == (1, 40)
+
+
+def test_graph_rerecorded_when_targets_grow(mods):
+    """set_targets switching on modes beyond the active band rebuilds the fine
+    grid: the recorded iteration is stale and must be recorded again."""
+    LBFGS_CPP, P = mods
+    N, kp = 500, 12
+    Sk, V = stealthy(kp, 2, 4)
+    pot = P.NUwU(np.ones(N), kp, Sk, V, np.ones(2), 1e-6, precision="fp32")
+    x0 = np.random.default_rng(3).uniform(0, 1, 2 * N)
+    ga = LBFGS_CPP(x0, pot, tol=0.0, M=4, use_graph=True)
+    eb = LBFGS_CPP(x0, pot, tol=0.0, M=4, use_graph=False)
+    for it in range(8):
+        same(ga.one_iteration(), eb.one_iteration())
+    nf0 = pot.plan_info()["nf"]
+    Sk2, V2 = stealthy(kp, 2, 10)
+    pot.set_targets(Sk2, V2)
+    assert pot.plan_info()["nf"] != nf0
+    for it in range(8):
+        same(ga.one_iteration(), eb.one_iteration())
+    assert ga.graph_stats() == (2, 16)
+
+
+def test_graph_off_for_unrecordable_potentials(mods):
+    """Potentials without an enqueue-only evaluation (here a Python one) run
+    the launch-by-launch path with the same results."""
+    LBFGS_CPP, P = mods
+
+    class Quartic:
+        def get_energy_gradient(self, x):
+            x = np.asarray(x)
+            return float(np.sum(x ** 4)), 4 * x ** 3
+
+        def get_energy(self, x):
+            return float(np.sum(np.asarray(x) ** 4))
+
+    x0 = np.random.default_rng(5).uniform(-1, 1, 50)
+    ga = LBFGS_CPP(x0, Quartic(), tol=0.0, M=4, use_graph=True)
+    eb = LBFGS_CPP(x0, Quartic(), tol=0.0, M=4, use_graph=False)
+    for it in range(20):
+        same(ga.one_iteration(), eb.one_iteration())
+    assert ga.graph_stats() == (0, 0)
