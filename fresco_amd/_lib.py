@@ -105,6 +105,15 @@ SIGNATURES = {
     ),
     "fresco_opt_get_x": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "fresco_opt_graph_stats": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p]),
+    "fresco_structure_factor": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, c_double_p, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "fresco_radial_profile": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int, c_int64_p, c_double_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "fresco_opt_get_g": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),

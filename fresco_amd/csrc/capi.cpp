@@ -14,6 +14,7 @@
 #include "optim.hpp"
 #include "potential.hpp"
 #include "prof.hpp"
+#include "structure.hpp"
 
 struct fresco_pot {
   std::shared_ptr<fresco_hip::Potential> impl;
@@ -404,6 +405,22 @@ int fresco_prof_get(int stage, const char** name, double* total_ms, int64_t* cou
     fresco_hip::prof_get(stage, total_ms, &c);
     *count = c;
     if (name) *name = fresco_hip::stage_name(stage);
+  });
+}
+
+int fresco_structure_factor(const double* x, const double* c, int64_t n, int d, const double* L, int64_t nk,
+                            double eps, int precision, double* Sk, double* gr, void* stream) {
+  return guarded([&]() {
+    if (!L) throw fresco_hip::InvalidArgument("null pointer");
+    fresco_hip::structure_factor(x, c, n, d, L, nk, eps, precision, Sk, gr, static_cast<hipStream_t>(stream));
+  });
+}
+
+int fresco_radial_profile(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                          int64_t* counts, void* stream) {
+  return guarded([&]() {
+    if (!shape || !center) throw fresco_hip::InvalidArgument("null pointer");
+    fresco_hip::radial_profile(data, d, shape, center, sums, counts, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -251,6 +251,33 @@ int fresco_pot_plan_info(fresco_pot* pot, int64_t* nf, int* w, double* beta, int
 int fresco_pot_destroy(fresco_pot* pot);
 
 /*
+ * Structure factor and pair correlation of a pattern: replaces the finufft
+ * nufft{2,3}d1 call and the numpy inverse FFT of export_structure
+ * (fresco/utils/utils.py:38-46).
+ *   x[n*d]: device float64 positions; c[n]: device float64 weights (the
+ *   caller's ones or normalised volumes, utils.py:26-35); L[d]: host.
+ *   Sk[nk^d] (device): |sum_j c_j e^{i k.x_j}|^2 / n, x = 2 pi p / L,
+ *   k_a = i_a - nk/2, first index slowest (x), as the Python finufft returns.
+ *   gr[nk^d] (device, may be NULL): fftshift(real(ifftn(ifftshift(Sk) - 1)))
+ *   * nk^d * prod(L) / n.
+ *   eps: NUFFT tolerance (utils.py:16 tol); precision 0: float64, 1: float32.
+ *   Returns after the results are complete.
+ */
+int fresco_structure_factor(const double* x, const double* c, int64_t n, int d, const double* L, int64_t nk,
+                            double eps, int precision, double* Sk, double* gr, void* stream);
+
+/*
+ * Shell sums of radial_profile (fresco/utils/utils.py:138-152).
+ *   data: device float64 grid, C order, shape[d], d <= 3; center[d]: host.
+ *   sums[b], counts[b] (device, b = 0 .. shape[0]/2 + 1): sum and number of
+ *   cells with floor(|i - center|) = b; cells beyond R = shape[0]/2 fold into
+ *   b = R + 1.  The profile is sums / counts over the shells present.
+ *   Returns after the results are complete.
+ */
+int fresco_radial_profile(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                          int64_t* counts, void* stream);
+
+/*
  * Sum of potentials over the same coordinates.
  * Replaces fresco::CombinedPotential (source/fresco/combine_potentials.hpp:16-59),
  * fresco/potentials/combine_potentials.pyx:10-30.  The children stay alive while
