@@ -117,6 +117,7 @@ SIGNATURES = {
     "fresco_opt_get_g": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),
+    "fresco_prof_enable_stages": (ctypes.c_int, [ctypes.c_uint]),
     "fresco_prof_reset": (ctypes.c_int, []),
     "fresco_prof_get": (
         ctypes.c_int,
@@ -128,12 +129,27 @@ N_STAGES = 8
 
 OPT_TOL, OPT_MAXSTEP, OPT_MAX_ITER, OPT_IPRINT, OPT_VERBOSITY, OPT_H0, OPT_MAX_F_RISE, OPT_USE_RELATIVE_F = range(8)
 OPT_USE_GRAPH = 8
+OPT_LBFGS_FORM = 9
 
 EG_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, c_double_p, c_double_p, c_double_p, ctypes.c_void_p)
 
 
 def prof_enable(on=True):
     check(load().fresco_prof_enable(1 if on else 0))
+
+
+def prof_enable_stages(names):
+    """Time only the named stages (e.g. ["spread"])."""
+    lib = load()
+    mask = 0
+    for st in range(N_STAGES):
+        name = ctypes.c_char_p()
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        check(lib.fresco_prof_get(st, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(cnt)))
+        if name.value.decode() in names:
+            mask |= 1 << st
+    check(lib.fresco_prof_enable_stages(mask))
 
 
 def prof_reset():

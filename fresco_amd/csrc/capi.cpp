@@ -242,6 +242,10 @@ int fresco_opt_set_param(fresco_opt* opt, int param, double value) {
       case FRESCO_OPT_IPRINT: o->set_iprint((int)value); break;
       case FRESCO_OPT_VERBOSITY: o->set_verbosity((int)value); break;
       case FRESCO_OPT_USE_GRAPH: o->set_use_graph(value != 0.0); break;
+      case FRESCO_OPT_LBFGS_FORM:
+        if (!opt->lbfgs) throw fresco_hip::InvalidArgument("parameter only applies to LBFGS");
+        opt->lbfgs->set_form((int)value);
+        break;
       case FRESCO_OPT_H0:
       case FRESCO_OPT_MAX_F_RISE:
       case FRESCO_OPT_USE_RELATIVE_F:
@@ -392,6 +396,10 @@ int fresco_pot_plan_info(fresco_pot* pot, int64_t* nf, int* w, double* beta, int
 
 int fresco_prof_enable(int on) {
   return guarded([&]() { fresco_hip::prof_set(on != 0); });
+}
+
+int fresco_prof_enable_stages(unsigned mask) {
+  return guarded([&]() { fresco_hip::prof_set_mask(mask); });
 }
 
 int fresco_prof_reset(void) {

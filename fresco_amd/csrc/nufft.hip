@@ -722,7 +722,7 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
   float best = 1e30f;
   Cand pick = cands[0];
   tuned = true;  // the launches below dispatch on mode / halo_parts directly
-  const bool prof = prof_enabled();
+  const unsigned prof = prof_mask();
   prof_set(false);
   for (const Cand& c : cands) {
     mode = c.mode;
@@ -739,7 +739,7 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
       pick = c;
     }
   }
-  prof_set(prof);
+  prof_set_mask(prof);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   mode = pick.mode;

@@ -454,7 +454,32 @@ __global__ void __launch_bounds__(RB)
 }
 
 // energy-rise test of the trial (lbfgs.cpp:195-204); on acceptance (or
-// force) rho, H0, f and k as in lbfgs.cpp:86-104
+// force) rho, H0, f and k as in lbfgs.cpp:86-104.  Returns the decision.
+__device__ bool lb_commit_scalars(double* __restrict__ dsc, int M, int force, double ys, double yy, double gg) {
+  const double f = dsc[LS_F], fnew = dsc[LS_FNEW];
+  double df = fnew - f;
+  if (dsc[LS_REL] != 0.0) {
+    double absf = 1e-100;
+    if (f != 0) absf = fabs(f);
+    df /= absf;
+  }
+  const bool accept = force != 0 || df < dsc[LS_MAXRISE];
+  dsc[LS_GG] = gg;
+  dsc[LS_YS] = ys;
+  dsc[LS_YY] = yy;
+  dsc[LS_ACCEPT] = accept ? 1.0 : 0.0;
+  if (accept) {
+    const int k = (int)dsc[LS_K];
+    const double ys1 = ys == 0. ? 1. : ys;
+    const double yy1 = yy == 0. ? 1. : yy;
+    dsc[LS_RHO + k % M] = 1. / ys1;
+    dsc[LS_H0] = ys1 / yy1;
+    dsc[LS_F] = fnew;
+    dsc[LS_K] = (double)(k + 1);
+  }
+  return accept;
+}
+
 __global__ void __launch_bounds__(RB)
     k_lb_commit(const double* __restrict__ part, int nb, double* __restrict__ dsc, int M, int force) {
   __shared__ double sh[RB];
@@ -467,28 +492,274 @@ __global__ void __launch_bounds__(RB)
   ys = block_sum(ys, sh);
   yy = block_sum(yy, sh);
   gg = block_sum(gg, sh);
+  if (threadIdx.x == 0) lb_commit_scalars(dsc, M, force, ys, yy, gg);
+}
+
+// ---------------------------------------------------------------------------
+// Vector-free two-loop recursion (Chen, Wang & Zhou, "Large-scale L-BFGS
+// using MapReduce", NIPS 2014).  The step is a combination of the basis
+// {s_q, y_q (filled slots), g}; the two-loop recursion of lbfgs.cpp:128-160
+// runs on its 2M+1 coefficients with the basis' dot products (one thread),
+// and the step is formed in one pass over the basis.  The dot products of a
+// new pair with every slot and of every slot with g come out of the memory
+// pass that writes the pair, so an iteration reads each memory vector twice
+// (vs about five times in the slot-by-slot recursion).
+// ---------------------------------------------------------------------------
+
+constexpr int VL_MAXM = 16;
+
+// Gram storage: ss[p*M+q] = s_p.s_q, sy[p*M+q] = s_p.y_q, yy[p*M+q] = y_p.y_q,
+// sg[q] = s_q.g, yg[q] = y_q.g (g the gradient at the committed x)
+struct VlGram {
+  double* ss;
+  double* sy;
+  double* yy;
+  double* sg;
+  double* yg;
+};
+
+// slots holding a pair after k memory updates
+__device__ __forceinline__ unsigned vl_mask(int k, int M) {
+  const int c = k < M ? k : M;
+  return c >= 32 ? 0xffffffffu : ((1u << c) - 1u);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// 64-lane sums of V values, then the 4 wavefronts in LDS;
+// part[v * RMAXB + blockIdx.x] = block sum of v
+template <int V>
+__device__ __forceinline__ void vl_block_store(double (&v)[V], double* __restrict__ part) {
+  __shared__ double sh[RB / 64][V];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const double x = wave_sum(v[i]);
+    if (lane == 0) sh[wv][i] = x;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < V; i += RB) {
+    double t = 0;
+#pragma unroll
+    for (int w = 0; w < RB / 64; ++w) t += sh[w][i];
+    part[(int64_t)i * RMAXB + blockIdx.x] = t;
+  }
+}
+
+// tot[v] = sum of the nb block partials of value v (one workgroup per value)
+__global__ void __launch_bounds__(RB) k_vl_totals(const double* __restrict__ part, int nb, double* __restrict__ tot) {
+  __shared__ double sh[RB];
+  const int v = blockIdx.x;
+  double x = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) x += part[(int64_t)v * RMAXB + b];
+  x = block_sum(x, sh);
+  if (threadIdx.x == 0) tot[v] = x;
+}
+
+// coefficients delta = [s_0..s_15, y_0..y_15, g] of the step: one wavefront,
+// lane l owns basis vector l (s_l for l < M, y_{l-M} for l < 2M, g for 2M);
+// the Gram entries are staged in LDS first
+__global__ void __launch_bounds__(64)
+    k_vl_coef(const double* __restrict__ dsc, VlGram G, double* __restrict__ delta, int M) {
+  __shared__ double ss[VL_MAXM * VL_MAXM], sy[VL_MAXM * VL_MAXM], yy[VL_MAXM * VL_MAXM];
+  __shared__ double sg[VL_MAXM], yg[VL_MAXM], rho[VL_MAXM], alpha[VL_MAXM];
+  const int l = threadIdx.x;
+  for (int i = l; i < M * M; i += 64) {
+    ss[i] = G.ss[i];
+    sy[i] = G.sy[i];
+    yy[i] = G.yy[i];
+  }
+  if (l < M) {
+    sg[l] = G.sg[l];
+    yg[l] = G.yg[l];
+    rho[l] = dsc[LS_RHO + l];
+  }
+  __syncthreads();
+  const int k = (int)dsc[LS_K];
+  const int cnt = k < M ? k : M;
+  const unsigned mask = vl_mask(k, M);
+  const bool is_s = l < M, is_y = l >= M && l < 2 * M, is_g = l == 2 * M;
+  const int q = is_s ? l : l - M;
+  const bool on = is_g || ((is_s || is_y) && ((mask >> q) & 1u));
+  double dl = 0.0;
+  if (k == 0) {
+    // lbfgs.cpp:110-125
+    double gn = sqrt(dsc[LS_GG]);
+    if (gn > 1.) gn = 1. / gn;
+    if (is_g) dl = -gn * dsc[LS_H0];
+  } else {
+    if (is_g) dl = 1.0;
+    for (int j = k - 1; j >= k - cnt; --j) {  // lbfgs.cpp:133-143
+      const int i = j % M;
+      double term = 0.0;
+      if (on) term = dl * (is_g ? sg[i] : (is_s ? ss[i * M + q] : sy[i * M + q]));
+      const double a = rho[i] * wave_sum(term);
+      if (l == 0) alpha[i] = a;
+      if (is_y && q == i) dl -= a;
+    }
+    dl *= -dsc[LS_H0];  // lbfgs.cpp:144-147
+    __syncthreads();
+    for (int j = k - cnt; j < k; ++j) {  // lbfgs.cpp:149-160
+      const int i = j % M;
+      double term = 0.0;
+      if (on) term = dl * (is_g ? yg[i] : (is_s ? sy[q * M + i] : yy[i * M + q]));
+      const double t = wave_sum(term);
+      if (is_s && q == i) dl -= alpha[i] + rho[i] * t;
+    }
+  }
+  if (is_s) delta[q] = dl;
+  if (is_y) delta[VL_MAXM + q] = dl;
+  if (is_g) delta[2 * VL_MAXM] = dl;
+}
+
+// step = sum of delta times the basis; xold = x, gold = g; partial sums of
+// step.g and step.step (k_lb_line finishes them).  Loads of slots not filled
+// yet are issued anyway (always in bounds) and their values discarded, so all
+// loads of an element can be in flight together.
+template <int M>
+__global__ void __launch_bounds__(RB)
+    k_vl_step(const double* __restrict__ x, const double* __restrict__ g, double* __restrict__ xold,
+              double* __restrict__ gold, double* __restrict__ step, const double* __restrict__ sb,
+              const double* __restrict__ yb, int64_t n, const double* __restrict__ delta,
+              const double* __restrict__ dsc, double* __restrict__ part) {
+  __shared__ double sh[RB];
+  const unsigned mask = vl_mask((int)dsc[LS_K], M);
+  double ds[M], dy[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const bool on = (mask >> q) & 1u;
+    ds[q] = on ? delta[q] : 0.0;
+    dy[q] = on ? delta[VL_MAXM + q] : 0.0;
+  }
+  const double dg = delta[2 * VL_MAXM];
+  double a0 = 0, a1 = 0;
+  for (int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x; t < n; t += (int64_t)gridDim.x * RB) {
+    const double gt = g[t];
+    const double xt = x[t];
+    double sv[M], yv[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      sv[q] = sb[(int64_t)q * n + t];
+      yv[q] = yb[(int64_t)q * n + t];
+    }
+    double st = dg * gt;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const bool on = (mask >> q) & 1u;
+      st += on ? ds[q] * sv[q] + dy[q] * yv[q] : 0.0;
+    }
+    step[t] = st;
+    xold[t] = xt;
+    gold[t] = gt;
+    a0 += st * gt;
+    a1 += st * st;
+  }
+  a0 = block_sum(a0, sh);
+  a1 = block_sum(a1, sh);
   if (threadIdx.x == 0) {
-    const double f = dsc[LS_F], fnew = dsc[LS_FNEW];
-    double df = fnew - f;
-    if (dsc[LS_REL] != 0.0) {
-      double absf = 1e-100;
-      if (f != 0) absf = fabs(f);
-      df /= absf;
+    part[blockIdx.x] = a0;
+    part[RMAXB + blockIdx.x] = a1;
+  }
+}
+
+// memory pass: s, y of slot k % M; ys, yy, g.g; and for every slot filled
+// after this update, its dot products with g and with the new pair
+template <int M>
+__global__ void __launch_bounds__(RB)
+    k_vl_memory(const double* __restrict__ x, const double* __restrict__ xold, const double* __restrict__ g,
+                const double* __restrict__ gold, double* __restrict__ sb, double* __restrict__ yb, int64_t n,
+                const double* __restrict__ dsc, double* __restrict__ part) {
+  constexpr int V = 6 * M + 3;
+  const int k = (int)dsc[LS_K];
+  const int kl = k % M;
+  const unsigned mask = vl_mask(k + 1, M);
+  double acc[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) acc[i] = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x; t < n; t += (int64_t)gridDim.x * RB) {
+    const double gi = g[t];
+    const double snew = x[t] - xold[t];
+    const double ynew = gi - gold[t];
+    double sv[M], yv[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      sv[q] = sb[(int64_t)q * n + t];
+      yv[q] = yb[(int64_t)q * n + t];
     }
-    const bool accept = force != 0 || df < dsc[LS_MAXRISE];
-    dsc[LS_GG] = gg;
-    dsc[LS_YS] = ys;
-    dsc[LS_YY] = yy;
-    dsc[LS_ACCEPT] = accept ? 1.0 : 0.0;
-    if (accept) {
-      const int k = (int)dsc[LS_K];
-      const double ys1 = ys == 0. ? 1. : ys;
-      const double yy1 = yy == 0. ? 1. : yy;
-      dsc[LS_RHO + k % M] = 1. / ys1;
-      dsc[LS_H0] = ys1 / yy1;
-      dsc[LS_F] = fnew;
-      dsc[LS_K] = (double)(k + 1);
+    acc[0] += ynew * snew;
+    acc[1] += ynew * ynew;
+    acc[2] += gi * gi;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const bool on = (mask >> q) & 1u;
+      const double sq = q == kl ? snew : (on ? sv[q] : 0.0);
+      const double yq = q == kl ? ynew : (on ? yv[q] : 0.0);
+      acc[3 + 6 * q] += sq * gi;
+      acc[4 + 6 * q] += yq * gi;
+      acc[5 + 6 * q] += snew * sq;
+      acc[6 + 6 * q] += snew * yq;
+      acc[7 + 6 * q] += ynew * sq;
+      acc[8 + 6 * q] += ynew * yq;
     }
+    sb[(int64_t)kl * n + t] = snew;
+    yb[(int64_t)kl * n + t] = ynew;
+  }
+  vl_block_store<V>(acc, part);
+}
+
+// commit of the memory pass from the totals: the scalars as k_lb_commit, and
+// on acceptance the Gram rows of the new pair and every slot's products with g
+__global__ void k_vl_commit(const double* __restrict__ tot, double* __restrict__ dsc, VlGram G, int M, int force) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int k = (int)dsc[LS_K];
+  const int kl = k % M;
+  const unsigned mask = vl_mask(k + 1, M);
+  if (!lb_commit_scalars(dsc, M, force, tot[0], tot[1], tot[2])) return;
+  for (int q = 0; q < M; ++q) {
+    if (!((mask >> q) & 1u)) continue;
+    const double* a = tot + 3 + 6 * q;
+    G.sg[q] = a[0];
+    G.yg[q] = a[1];
+    G.ss[kl * M + q] = a[2];
+    G.ss[q * M + kl] = a[2];
+    G.sy[kl * M + q] = a[3];  // s_new . y_q
+    G.sy[q * M + kl] = a[4];  // s_q . y_new
+    G.yy[kl * M + q] = a[5];
+    G.yy[q * M + kl] = a[5];
+  }
+}
+
+// s_q.g and y_q.g of the filled slots (after the gradient is replaced)
+template <int M>
+__global__ void __launch_bounds__(RB)
+    k_vl_gdots(const double* __restrict__ g, const double* __restrict__ sb, const double* __restrict__ yb, int64_t n,
+               double* __restrict__ part) {
+  constexpr int V = 2 * M;
+  double acc[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) acc[i] = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x; t < n; t += (int64_t)gridDim.x * RB) {
+    const double gi = g[t];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      acc[2 * q] += sb[(int64_t)q * n + t] * gi;
+      acc[2 * q + 1] += yb[(int64_t)q * n + t] * gi;
+    }
+  }
+  vl_block_store<V>(acc, part);
+}
+
+__global__ void k_vl_gdots_set(const double* __restrict__ tot, const double* __restrict__ dsc, VlGram G, int M) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const unsigned mask = vl_mask((int)dsc[LS_K], M);
+  for (int q = 0; q < M; ++q) {
+    if (!((mask >> q) & 1u)) continue;
+    G.sg[q] = tot[2 * q];
+    G.yg[q] = tot[2 * q + 1];
   }
 }
 
@@ -513,6 +784,24 @@ LBFGS::LBFGS(std::shared_ptr<Potential> pot, const double* x0, int64_t n, double
   init[LS_MAXRISE] = max_f_rise_;
   init[LS_REL] = use_relative_f_ ? 1.0 : 0.0;
   dsc_.upload(init.data(), init.size(), stream_);
+  set_form(0);
+}
+
+void LBFGS::set_form(int form) {
+  if (form < 0 || form > 2) throw InvalidArgument("LBFGS form must be 0 (auto), 1 (two-loop) or 2 (vector-free)");
+  if (iter_number_ > 0 || k_ > 0) throw InvalidArgument("the LBFGS form can only be chosen before the first iteration");
+  if (form == 2 && M_ > VL_MAXM)
+    throw InvalidArgument("the vector-free form takes M <= " + std::to_string(VL_MAXM));
+  vl_ = form == 2 || (form == 0 && n_ >= VL_MIN_N && M_ <= VL_MAXM);
+  if (vl_ && !gram_.get()) {
+    gram_.allocate(3 * (size_t)M_ * M_ + 2 * (size_t)M_);
+    delta_.allocate(2 * VL_MAXM + 1 + 6 * VL_MAXM + 3);  // coefficients, then the pass totals
+    part_vl_.allocate((size_t)(6 * VL_MAXM + 3) * RMAXB);
+  }
+  if (gexec_) {
+    FRESCO_HIP_CHECK(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+  }
 }
 
 LBFGS::~LBFGS() {
@@ -559,6 +848,8 @@ void LBFGS::set_func_gradient(double f, const double* grad_host) {
   hipLaunchKernelGGL(k_lb_set, dim3(1), dim3(64), 0, stream_, dsc_.get(), (int)LS_F, f_, (int)LS_GG, gg_, -1, 0.0, -1,
                      0.0);
   FRESCO_HIP_CHECK(hipGetLastError());
+  // the vector-free recursion keeps the memory's products with g
+  if (vl_ && k_ > 0) enqueue_gdots(stream_);
 }
 
 void LBFGS::read_scalars() {
@@ -586,11 +877,74 @@ void LBFGS::enqueue_trial(bool dev_factor, double host_factor, hipStream_t s) {
   }
 }
 
+namespace {
+
+VlGram vl_gram(double* base, int M) {
+  VlGram G;
+  G.ss = base;
+  G.sy = base + (size_t)M * M;
+  G.yy = base + 2 * (size_t)M * M;
+  G.sg = base + 3 * (size_t)M * M;
+  G.yg = G.sg + M;
+  return G;
+}
+
+// launches F<M> for the memory length M (1..16) of this optimizer
+#define FRESCO_VL_DISPATCH(M, CALL) \
+  switch (M) {                      \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 3: CALL(3); break;         \
+    case 4: CALL(4); break;         \
+    case 5: CALL(5); break;         \
+    case 6: CALL(6); break;         \
+    case 7: CALL(7); break;         \
+    case 8: CALL(8); break;         \
+    case 9: CALL(9); break;         \
+    case 10: CALL(10); break;       \
+    case 11: CALL(11); break;       \
+    case 12: CALL(12); break;       \
+    case 13: CALL(13); break;       \
+    case 14: CALL(14); break;       \
+    case 15: CALL(15); break;       \
+    default: CALL(16); break;       \
+  }
+
+}  // namespace
+
 void LBFGS::enqueue_commit(int force, hipStream_t s) {
   const unsigned nb = nblocks(n_);
+  if (vl_) {
+    const VlGram G = vl_gram(gram_.get(), M_);
+    double* part = part_vl_.get();
+    double* tot = delta_.get() + 2 * VL_MAXM + 1;
+#define FRESCO_VL_MEM(MM)                                                                                     \
+  hipLaunchKernelGGL(k_vl_memory<MM>, dim3(nb), dim3(RB), 0, s, x_.get(), xold_.get(), g_.get(), gold_.get(), \
+                     s_.get(), y_.get(), n_, dsc_.get(), part)
+    FRESCO_VL_DISPATCH(M_, FRESCO_VL_MEM)
+#undef FRESCO_VL_MEM
+    hipLaunchKernelGGL(k_vl_totals, dim3(6 * M_ + 3), dim3(RB), 0, s, part, (int)nb, tot);
+    hipLaunchKernelGGL(k_vl_commit, dim3(1), dim3(64), 0, s, tot, dsc_.get(), G, M_, force);
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_lb_memory, dim3(nb), dim3(RB), 0, s, x_.get(), xold_.get(), g_.get(), gold_.get(), s_.get(),
                      y_.get(), n_, dsc_.get(), M_, ops_.part.get());
   hipLaunchKernelGGL(k_lb_commit, dim3(1), dim3(RB), 0, s, ops_.part.get(), (int)nb, dsc_.get(), M_, force);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+void LBFGS::enqueue_gdots(hipStream_t s) {
+  const unsigned nb = nblocks(n_);
+  const VlGram G = vl_gram(gram_.get(), M_);
+  double* part = part_vl_.get();
+  double* tot = delta_.get() + 2 * VL_MAXM + 1;
+#define FRESCO_VL_GD(MM) \
+  hipLaunchKernelGGL(k_vl_gdots<MM>, dim3(nb), dim3(RB), 0, s, g_.get(), s_.get(), y_.get(), n_, part)
+  FRESCO_VL_DISPATCH(M_, FRESCO_VL_GD)
+#undef FRESCO_VL_GD
+  hipLaunchKernelGGL(k_vl_totals, dim3(2 * M_), dim3(RB), 0, s, part, (int)nb, tot);
+  hipLaunchKernelGGL(k_vl_gdots_set, dim3(1), dim3(64), 0, s, tot, dsc_.get(), G, M_);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 
@@ -600,6 +954,20 @@ void LBFGS::enqueue_iteration(hipStream_t s) {
   const unsigned nb = nblocks(n_);
   double* part = ops_.part.get();
   double* dsc = dsc_.get();
+  if (vl_) {
+    const VlGram G = vl_gram(gram_.get(), M_);
+    hipLaunchKernelGGL(k_vl_coef, dim3(1), dim3(64), 0, s, dsc, G, delta_.get(), M_);
+#define FRESCO_VL_STEP(MM)                                                                                   \
+  hipLaunchKernelGGL(k_vl_step<MM>, dim3(nb), dim3(RB), 0, s, x_.get(), g_.get(), xold_.get(), gold_.get(), \
+                     step_.get(), s_.get(), y_.get(), n_, delta_.get(), dsc, part)
+    FRESCO_VL_DISPATCH(M_, FRESCO_VL_STEP)
+#undef FRESCO_VL_STEP
+    hipLaunchKernelGGL(k_lb_line, dim3(1), dim3(RB), 0, s, part, (int)nb, dsc);
+    FRESCO_HIP_CHECK(hipGetLastError());
+    enqueue_trial(true, 0.0, s);
+    enqueue_commit(0, s);
+    return;
+  }
   hipLaunchKernelGGL(k_lb_begin, dim3(eb), dim3(256), 0, s, x_.get(), g_.get(), xold_.get(), gold_.get(), step_.get(),
                      dsc, n_);
   for (int jj = 0; jj < M_; ++jj) {

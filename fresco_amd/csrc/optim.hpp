@@ -113,6 +113,12 @@ class LBFGS final : public GradientOptimizer {
   void set_use_relative_f(int v);
   double get_H0() const override { return H0_; }
   void set_use_graph(bool on) override { use_graph_ = on; }
+  // 0: vector-free recursion from VL_MIN_N unknowns on (M <= 16), else the
+  // slot-by-slot two-loop recursion; 1: always slot by slot; 2: always
+  // vector-free (M <= 16).  Only before the first iteration.
+  void set_form(int form);
+  bool vector_free() const { return vl_; }
+  static constexpr int64_t VL_MIN_N = 1 << 16;
   void graph_stats(int64_t* captures, int64_t* replays) const override {
     *captures = captures_;
     *replays = replays_;
@@ -124,6 +130,7 @@ class LBFGS final : public GradientOptimizer {
   void enqueue_iteration(hipStream_t s);
   void enqueue_trial(bool dev_factor, double host_factor, hipStream_t s);
   void enqueue_commit(int force, hipStream_t s);
+  void enqueue_gdots(hipStream_t s);
   bool run_graph();
 
   int M_;
@@ -133,6 +140,8 @@ class LBFGS final : public GradientOptimizer {
   int k_ = 0;
   double inv_sqrt_size_ = 1.0;
   DeviceBuffer<double> xold_, gold_, step_, s_, y_, alpha_, dsc_;
+  bool vl_ = false;                             // vector-free recursion
+  DeviceBuffer<double> gram_, delta_, part_vl_;  // its Gram matrix, coefficients, partial sums
   double* hsc_ = nullptr;  // pinned mirror of the device scalars
   bool use_graph_ = true;
   hipStream_t cap_ = nullptr;

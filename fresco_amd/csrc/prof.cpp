@@ -15,6 +15,7 @@ struct Pending {
 };
 std::mutex g_mu;
 bool g_on = false;
+unsigned g_mask = 0;
 std::vector<hipEvent_t> g_pool;
 std::vector<Pending> g_pending;
 hipEvent_t g_open[ST_COUNT] = {};
@@ -40,9 +41,16 @@ const char* stage_name(int st) {
 
 bool prof_enabled() { return g_on; }
 
-void prof_set(bool on) {
+unsigned prof_mask() { return g_mask; }
+
+bool prof_stage_on(int st) { return g_on && ((g_mask >> st) & 1u); }
+
+void prof_set(bool on) { prof_set_mask(on ? (1u << ST_COUNT) - 1u : 0u); }
+
+void prof_set_mask(unsigned m) {
   std::lock_guard<std::mutex> l(g_mu);
-  g_on = on;
+  g_mask = m & ((1u << ST_COUNT) - 1u);
+  g_on = g_mask != 0;
 }
 
 void prof_reset() {

@@ -10,8 +10,11 @@ namespace fresco_hip {
 enum Stage { ST_SORT = 0, ST_SPREAD, ST_FFT_FWD, ST_ZERO, ST_LOSS, ST_FFT_INV, ST_INTERP, ST_MERGE, ST_COUNT };
 
 const char* stage_name(int st);
-bool prof_enabled();
-void prof_set(bool on);
+bool prof_enabled();             // any stage timed
+bool prof_stage_on(int st);      // this stage timed
+void prof_set(bool on);          // all stages / none
+void prof_set_mask(unsigned m);  // the stages whose bit is set
+unsigned prof_mask();
 void prof_reset();
 void prof_begin(int st, hipStream_t s);
 void prof_end(int st, hipStream_t s);
@@ -24,7 +27,7 @@ struct ProfScope {
   int st;
   hipStream_t s;
   bool on;
-  ProfScope(int st_, hipStream_t s_) : st(st_), s(s_), on(prof_enabled()) {
+  ProfScope(int st_, hipStream_t s_) : st(st_), s(s_), on(prof_stage_on(st_)) {
     if (on) prof_begin(st, s);
   }
   ~ProfScope() {

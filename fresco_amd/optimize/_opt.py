@@ -131,11 +131,16 @@ class LBFGS_CPP(GradientOptimizer):
     use_graph (not in the reference): replay each iteration as one recorded
     HIP graph when the potential can be recorded (NUwU, batched NUwU);
     False launches the same kernels one by one, with identical results.
+    recursion (not in the reference): "auto" (vector-free from 65536
+    unknowns on when M <= 16), "two-loop" (lbfgs.cpp as written) or
+    "vector-free" (the recursion on the coefficients of the memory vectors
+    and g, Chen, Wang & Zhou 2014: the same step up to rounding, each memory
+    vector read twice per iteration instead of about five times).
     """
 
     def __init__(self, x0, potential, tol=1e-5, M=4, maxstep=0.1, maxErise=1e-4, H0=0.1, iprint=-1,
                  energy=None, gradient=None, nsteps=10000, verbosity=0, events=None, logger=None,
-                 rel_energy=False, use_graph=True):
+                 rel_energy=False, use_graph=True, recursion="auto"):
         x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).ravel())
         self._n = len(x)
         self.pot = as_cpp_potential(potential, verbose=verbosity > 0, ndof=self._n)
@@ -154,6 +159,10 @@ class LBFGS_CPP(GradientOptimizer):
         if rel_energy:
             self._set(_lib.OPT_USE_RELATIVE_F, 1)
         self._set(_lib.OPT_USE_GRAPH, 1 if use_graph else 0)
+        forms = {"auto": 0, "two-loop": 1, "vector-free": 2}
+        if recursion not in forms:
+            raise ValueError("recursion must be one of %s" % sorted(forms))
+        self._set(_lib.OPT_LBFGS_FORM, forms[recursion])
         if energy is not None and gradient is not None:
             g = np.ascontiguousarray(np.asarray(gradient, dtype=np.float64).ravel())
             _lib.check(_lib.load().fresco_opt_set_func_gradient(self._h, float(energy), _lib.dptr(g), len(g)))

@@ -55,6 +55,13 @@ typedef int (*fresco_eg_callback)(const double* x, double* grad, double* energy,
  * iteration as one recorded HIP graph when the potential supports it; 0
  * launches the same kernels one by one.  Both give identical results. */
 #define FRESCO_OPT_USE_GRAPH 8
+/* No counterpart in the reference: LBFGS two-loop recursion form, set
+ * before the first iteration.  0 (default): vector-free from 65536 unknowns
+ * on when M <= 16, else slot by slot; 1: slot by slot (lbfgs.cpp:128-160
+ * as written); 2: vector-free (Chen, Wang & Zhou 2014: the recursion on the
+ * coefficients of {s, y, g} with their dot products; same step up to
+ * rounding). */
+#define FRESCO_OPT_LBFGS_FORM 9
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* fresco_last_error(void);
@@ -354,6 +361,9 @@ int fresco_opt_destroy(fresco_opt* opt);
  * output is the optimizer's iprint line, lbfgs.cpp:56-62).
  */
 int fresco_prof_enable(int on);
+/* Time only the stages whose bit (1 << stage) is set (0 = off): fewer events
+ * on the stream when one kernel's duration is wanted. */
+int fresco_prof_enable_stages(unsigned mask);
 int fresco_prof_reset(void);
 int fresco_prof_get(int stage, const char** name, double* total_ms, int64_t* count);
 
