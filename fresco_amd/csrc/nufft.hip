@@ -409,7 +409,10 @@ __device__ __forceinline__ T wave_sum63(T v) {
 constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8, HW = 8;
 constexpr int HP_X = HT_X + HW - 1, HP_Y = HT_Y + HW - 1, HP_Z = HT_Z + HW - 1;
 constexpr int HS_X = 24;  // row stride: (ja, jb) lane pattern conflict-free for 4- and 8-byte cells
-constexpr int HT_ELEMS = HS_X * HP_Y * HP_Z;
+// plane stride in rows: HS_X * HS_Y cells is a multiple of 64 words, so the 8
+// z-cells of a lane pair up into read2st64 / write2st64 LDS instructions
+constexpr int HS_Y = 24;
+constexpr int HT_ELEMS = HS_X * HS_Y * HP_Z;
 
 template <typename T>
 struct SpreadDev {
@@ -1320,8 +1323,8 @@ __global__ void __launch_bounds__(64)
         uA[d] = (int)((sa >> (d * a.cbits)) & cmask) - t0[d];
         uB[d] = (int)((sb >> (d * a.cbits)) & cmask) - t0[d];
       }
-      T* ca = tile + (uA[2] * HP_Y + uA[1] + jb) * HS_X + uA[0] + ja;
-      T* cb = tile + (uB[2] * HP_Y + uB[1] + jb) * HS_X + uB[0] + ja;
+      T* ca = tile + (uA[2] * HS_Y + uA[1] + jb) * HS_X + uA[0] + ja;
+      T* cb = tile + (uB[2] * HS_Y + uB[1] + jb) * HS_X + uB[0] + ja;
       T wzA[HW], wzB[HW];
 #pragma unroll
       for (int jc = 0; jc < HW; ++jc) {
@@ -1329,9 +1332,9 @@ __global__ void __launch_bounds__(64)
         wzB[jc] = lane_read(kv, 48 + jc);
       }
 #pragma unroll
-      for (int jc = 0; jc < HW; ++jc) ca[jc * HP_Y * HS_X] += vA * wzA[jc];
+      for (int jc = 0; jc < HW; ++jc) ca[jc * HS_Y * HS_X] += vA * wzA[jc];
 #pragma unroll
-      for (int jc = 0; jc < HW; ++jc) cb[jc * HP_Y * HS_X] += vB * wzB[jc];
+      for (int jc = 0; jc < HW; ++jc) cb[jc * HS_Y * HS_X] += vB * wzB[jc];
     }
   }
   __syncthreads();
@@ -1362,7 +1365,7 @@ __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__
           const int ex = tx[gx * 4 + ix];
           if (ex < 0) break;
           const int64_t bin = (ex >> 6) + (int64_t)a.ntile[0] * ((ey >> 6) + (int64_t)a.ntile[1] * (ez >> 6));
-          const int64_t off = ((ez & 63) * HP_Y + (ey & 63)) * HS_X + (ex & 63);
+          const int64_t off = ((ez & 63) * HS_Y + (ey & 63)) * HS_X + (ex & 63);
           for (int s = 0; s < parts; ++s) acc += halo[(bin * parts + s) * HT_ELEMS + off];
         }
       }
