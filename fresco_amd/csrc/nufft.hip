@@ -234,23 +234,37 @@ __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, 
   cell = (int)ic;
 }
 
-template <typename K>
-__global__ void k_bin_keys(const double* __restrict__ x, int64_t n, MapDev m, K* __restrict__ key,
-                           uint32_t* __restrict__ idx, const uint32_t* __restrict__ pat, uint32_t nbins) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// bin key of every point (pattern * nbins + tile of its first support cell)
+// and, in the original order, its record (fine-grid offsets and weight) and
+// packed first cell, which the sort carries to the sorted order
+template <typename T>
+__global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m, uint32_t* __restrict__ key,
+                              uint32_t* __restrict__ idx, const uint32_t* __restrict__ pat, uint32_t nbins,
+                              const T* __restrict__ c, uint32_t* __restrict__ cell_out,
+                              typename Vec4<T>::type* __restrict__ rec_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint32_t b = pat ? pat[j] * nbins : 0u, mul = 1;
+  double offs[3] = {0, 0, 0};
+  uint32_t packed = 0;
   for (int a = 0; a < m.d; ++a) {
     int cell;
-    double off;
-    map_component(x[j * m.d + a], m, a, cell, off);
+    map_component(x[j * m.d + a], m, a, cell, offs[a]);
+    packed |= (uint32_t)cell << (a * m.cell_bits);
     int t = cell / m.tile[a];
     t = t < m.ntile[a] ? t : m.ntile[a] - 1;
     b += mul * (uint32_t)t;
     mul *= (uint32_t)m.ntile[a];
   }
-  key[j] = (K)b;
+  key[j] = b;
   idx[j] = (uint32_t)j;
+  typename Vec4<T>::type r;
+  r.x = (T)offs[0];
+  r.y = (T)offs[1];
+  r.z = (T)offs[2];
+  r.w = c ? c[j] : T(1);
+  cell_out[j] = packed;
+  rec_out[j] = r;
 }
 
 template <typename K>
@@ -266,29 +280,6 @@ __global__ void k_bin_start(const K* __restrict__ keys, int64_t n, int nbins, in
       hi = mid;
   }
   start[b] = (int)lo;
-}
-
-template <typename T>
-__global__ void k_permute(const double* __restrict__ x, int64_t n, MapDev m, const uint32_t* __restrict__ idx,
-                          const T* __restrict__ c, uint32_t* __restrict__ scell,
-                          typename Vec4<T>::type* __restrict__ rec) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  uint32_t j = idx[k];
-  double offs[3] = {0, 0, 0};
-  uint32_t packed = 0;
-  for (int a = 0; a < m.d; ++a) {
-    int cell;
-    map_component(x[(int64_t)j * m.d + a], m, a, cell, offs[a]);
-    packed |= (uint32_t)cell << (a * m.cell_bits);
-  }
-  typename Vec4<T>::type r;
-  r.x = (T)offs[0];
-  r.y = (T)offs[1];
-  r.z = (T)offs[2];
-  r.w = c ? c[j] : T(1);
-  scell[k] = packed;
-  rec[k] = r;
 }
 
 static MapDev make_map_dev(const GridGeom& g, const EsParams& es, const CoordMap& cm) {
@@ -315,13 +306,14 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   geom = g;
   es = e;
   n = npts;
-  // 16-bit bin keys when they fit: rocPRIM then runs its onesweep radix sort
   const int64_t tb = (int64_t)g.nbins * g.batch;
   idx_in.allocate(n);
   idx.allocate(n);
   bin_start.allocate(tb + 1);
   scell.allocate(n);
   rec.allocate(n);
+  scell_in.allocate(n);
+  rec_in.allocate(n);
   key.allocate(n);
   key_sorted.allocate(n);
   sorter.init(n, g.key_bits);
@@ -333,15 +325,20 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
   const unsigned thr = 256;
   if ((geom.batch > 1) != (pat != nullptr)) throw InvalidArgument("pattern indices needed exactly for batched plans");
   const int tb = geom.nbins * geom.batch;
-  hipLaunchKernelGGL(k_bin_keys<uint32_t>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(),
-                     idx_in.get(), pat, (uint32_t)geom.nbins);
+  // records computed once, in the original (coalesced) order, and carried to
+  // the sorted order by the sort's last pass
+  hipLaunchKernelGGL(k_bin_records<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(), idx_in.get(),
+                     pat, (uint32_t)geom.nbins, c, scell_in.get(), rec_in.get());
   FRESCO_HIP_CHECK(hipGetLastError());
-  sorter.sort(key.get(), idx_in.get(), key_sorted.get(), idx.get(), s);
+  SortPayload pl;
+  pl.rec_in = rec_in.get();
+  pl.rec_out = rec.get();
+  pl.rec_words = (int)(sizeof(typename Vec4<T>::type) / 4);
+  pl.cell_in = scell_in.get();
+  pl.cell_out = scell.get();
+  sorter.sort(key.get(), idx_in.get(), key_sorted.get(), idx.get(), s, &pl);
   hipLaunchKernelGGL(k_bin_start<uint32_t>, dim3(blocks_for(tb + 1, thr)), dim3(thr), 0, s, key_sorted.get(), n, tb,
                      bin_start.get());
-  FRESCO_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, idx.get(), c, scell.get(),
-                     rec.get());
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 

@@ -85,10 +85,12 @@ class SortedPoints {
   GridGeom geom;
   EsParams es;
   DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
-  BinSort sorter;                                       // stable LSD radix sort, 8-bit digits
+  BinSort sorter;                                       // stable LSD radix sort (sort.hip)
   DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
+  DeviceBuffer<uint32_t> scell_in;                      // the same in the original order
+  DeviceBuffer<R4> rec_in;
 };
 
 template <typename T>

@@ -14,6 +14,9 @@
 //                  and per-wavefront counts in LDS order the wavefronts.
 // Equal keys keep their input order, so the result equals any stable sort's;
 // no launch records a memory fill, so the sort can sit in a captured graph.
+// An optional payload (a record and a cell word per element, indexed by the
+// element's value) is moved by the last pass's scatter: with one pass and
+// values 0..n-1 its reads are coalesced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -110,7 +113,8 @@ __global__ void __launch_bounds__(RS_T)
     k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
                  uint32_t dmask, int dbits, int nbuck, const uint32_t* __restrict__ offs,
                  const uint32_t* __restrict__ tops, int nblk, uint32_t* __restrict__ kout,
-                 uint32_t* __restrict__ vout) {
+                 uint32_t* __restrict__ vout, const uint4* __restrict__ pin, uint4* __restrict__ pout, int pq,
+                 const uint32_t* __restrict__ cin, uint32_t* __restrict__ cout) {
   __shared__ uint32_t run[RS_MAXB];
   __shared__ uint32_t wc[RS_T / 64][RS_MAXB];
   const int t = threadIdx.x;
@@ -144,8 +148,13 @@ __global__ void __launch_bounds__(RS_T)
     if (valid) {
       uint32_t pos = run[dg] + (uint32_t)rank;
       for (int w = 0; w < wv; ++w) pos += wc[w][dg];
+      const uint32_t v = vin[e];
       kout[pos] = k;
-      vout[pos] = vin[e];
+      vout[pos] = v;
+      if (pout) {
+        for (int q = 0; q < pq; ++q) pout[(int64_t)pos * pq + q] = pin[(int64_t)v * pq + q];
+        cout[pos] = cin[v];
+      }
     }
     __syncthreads();
     for (int d = t; d < nbuck; d += RS_T) {
@@ -176,7 +185,9 @@ void BinSort::init(int64_t n, int key_bits) {
 }
 
 void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out,
-                   hipStream_t s) {
+                   hipStream_t s, const SortPayload* payload) {
+  if (payload && (payload->rec_words <= 0 || payload->rec_words % 4 != 0))
+    throw InvalidArgument("sort payload records must be whole 16-byte words");
   const uint32_t* kin = keys;
   const uint32_t* vin = vals;
   const int nbuck = 1 << dbits_;
@@ -192,8 +203,12 @@ void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_ou
     hipLaunchKernelGGL(k_rs_upsweep, dim3(nblk_), dim3(RS_T), 0, s, kin, n_, shift, dmask, nbuck, cnt_.get(), nblk_);
     hipLaunchKernelGGL(k_rs_scan_seg, dim3(nseg_), dim3(RS_T), 0, s, cnt_.get(), m, tops_.get());
     hipLaunchKernelGGL(k_rs_scan_top, dim3(1), dim3(RS_T), 0, s, tops_.get(), nseg_);
+    const bool last = p == passes_ - 1;
+    const SortPayload* pl = last ? payload : nullptr;
     hipLaunchKernelGGL(k_rs_scatter, dim3(nblk_), dim3(RS_T), 0, s, kin, vin, n_, shift, dmask, dbits_, nbuck,
-                       cnt_.get(), tops_.get(), nblk_, ko, vo);
+                       cnt_.get(), tops_.get(), nblk_, ko, vo, pl ? static_cast<const uint4*>(pl->rec_in) : nullptr,
+                       pl ? static_cast<uint4*>(pl->rec_out) : nullptr, pl ? pl->rec_words / 4 : 0,
+                       pl ? pl->cell_in : nullptr, pl ? pl->cell_out : nullptr);
     FRESCO_HIP_CHECK(hipGetLastError());
     kin = ko;
     vin = vo;

@@ -10,12 +10,25 @@
 
 namespace fresco_hip {
 
+// Records carried along by the sort: element i of the output gets
+// rec_in[v] (rec_words 32-bit words, a multiple of 4) and cell_in[v], v the
+// output value (the original index).  Moved in the last digit pass only.
+struct SortPayload {
+  const void* rec_in = nullptr;
+  void* rec_out = nullptr;
+  int rec_words = 4;
+  const uint32_t* cell_in = nullptr;
+  uint32_t* cell_out = nullptr;
+};
+
 class BinSort {
  public:
   // n pairs; keys are below 2^key_bits
   void init(int64_t n, int key_bits);
-  // keys / vals -> keys_out / vals_out (stable; inputs are not modified)
-  void sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s);
+  // keys / vals -> keys_out / vals_out (stable; inputs are not modified);
+  // with a payload, its records are placed in the output order too
+  void sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s,
+            const SortPayload* payload = nullptr);
 
  private:
   int64_t n_ = 0;
