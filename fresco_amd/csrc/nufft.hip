@@ -1581,12 +1581,24 @@ __global__ void __launch_bounds__(G8_THREADS)
   const int npts = end - beg;
   // every group runs the same number of iterations (DPP needs all lanes)
   const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
+  // the next iteration's point is loaded while this one is interpolated
+  int kn = beg + grp;
+  uint32_t sc_n = scell[kn < end ? kn : beg];
+  typename Vec4<T>::type r_n = rec[kn < end ? kn : beg];
+  uint32_t j_n = idx[kn < end ? kn : beg];
   for (int it = 0; it < iters; ++it) {
-    const int k = beg + it * (G8_THREADS / 8) + grp;
+    const int k = kn;
     const bool valid = k < end;
-    const int kk = valid ? k : beg;
-    const uint32_t sc = scell[kk];
-    const typename Vec4<T>::type r = rec[kk];
+    const uint32_t sc = sc_n;
+    const typename Vec4<T>::type r = r_n;
+    const uint32_t jout = j_n;
+    kn = k + G8_THREADS / 8;
+    if (it + 1 < iters) {
+      const int kq = kn < end ? kn : beg;
+      sc_n = scell[kq];
+      r_n = rec[kq];
+      j_n = idx[kq];
+    }
     int l0[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -1644,7 +1656,7 @@ __global__ void __launch_bounds__(G8_THREADS)
       vx = sum8_last(wxa * sA);
     }
     if (ja == 7 && valid) {
-      const int64_t j = idx[k];
+      const int64_t j = jout;
       const double cw = (double)r.w;
       if (DERIV) {
         out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
