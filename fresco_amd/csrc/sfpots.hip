@@ -152,6 +152,18 @@ class UwU final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    enqueue_eg(x, grad, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+
+  // energy_gradient without the wait (kernels only: recordable)
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return true; }
+
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
     forward(x, s);
     if (exact_) {
       hipLaunchKernelGGL((k_uwu_loss<T, 2>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), M_,
@@ -162,12 +174,11 @@ class UwU final : public Potential {
                          rhotot_.get(), f_.get(), 2.0, epart_.get(), cpart_.get());
     }
     FRESCO_HIP_CHECK(hipGetLastError());
-    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    reduce_partials(epart_.get(), nb_, e_out, s);
     fwd_.exec(f_.get(), c_.get(), s);  // FFTW sign -1 (uwu.hpp:49,187)
     hipLaunchKernelGGL(k_real_part<T>, dim3(rblocks(M_)), dim3(TB), 0, s, c_.get(), M_, 1.0,
                        exact_ ? cterm_.get() : nullptr, grad);
     FRESCO_HIP_CHECK(hipGetLastError());
-    return host_.fetch(e_.get(), s);
   }
 
   void get_rho(double* rho, hipStream_t s) override {
@@ -346,17 +357,28 @@ class UwNU final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    enqueue_eg(x, grad, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+
+  // energy_gradient without the wait (kernels only: recordable)
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return true; }
+
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
     forward(x, s);
     hipLaunchKernelGGL((k_pts_loss<T, 1>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), nk_, 1.0, 4.0,
                        0.0, f_.get(), epart_.get());
     FRESCO_HIP_CHECK(hipGetLastError());
-    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    reduce_partials(epart_.get(), nb_, e_out, s);
     // type-1 (-1) of f at the k points (uwnu.hpp:151)
     spreader_.spread_complex(pts_, f_.get(), fine_.get(), s);
     fwd_.exec(fine_.get(), fine2_.get(), s);
     hipLaunchKernelGGL(k_uwnu_extract<T>, dim3(rblocks(M_)), dim3(TB), 0, s, fine2_.get(), mg_, M_, grad);
     FRESCO_HIP_CHECK(hipGetLastError());
-    return host_.fetch(e_.get(), s);
   }
 
   void get_rho(double* rho, hipStream_t s) override {
@@ -576,16 +598,27 @@ class NUwNUDirect final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    enqueue_eg(x, grad, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+
+  // energy_gradient without the wait (kernels only: recordable)
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return true; }
+
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
     forward(x, s);
     // factor = (-4i/N) V D rho / S0 (nuwnu.hpp:228,247)
     hipLaunchKernelGGL((k_pts_loss<T, 1>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), nk_,
                        (double)N_, 0.0, -4.0 / (double)N_, f_.get(), epart_.get());
     FRESCO_HIP_CHECK(hipGetLastError());
-    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    reduce_partials(epart_.get(), nb_, e_out, s);
     hipLaunchKernelGGL(k_t3_adjoint<T>, dim3((unsigned)((N_ + TB - 1) / TB)), dim3(TB), 0, s, x, N_, k_.get(),
                        f_.get(), nk_, g_, grad);
     FRESCO_HIP_CHECK(hipGetLastError());
-    return host_.fetch(e_.get(), s);
   }
 
   void get_rho(double* rho, hipStream_t s) override {
@@ -879,12 +912,24 @@ class NUwNUType3 final : public Potential {
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    enqueue_eg(x, grad, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+
+  // energy_gradient without the wait (kernels only: recordable)
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    enqueue_eg(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return true; }
+
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
     forward(x, s);
     // f = (-4i/N) V D rho / S0 (nuwnu.hpp:228,247)
     hipLaunchKernelGGL((k_pts_loss<T, 1>), dim3(nb_), dim3(TB), 0, s, rho_.get(), Sk_.get(), V_.get(), nk_,
                        (double)N_, 0.0, -4.0 / (double)N_, fk_.get(), epart_.get());
     FRESCO_HIP_CHECK(hipGetLastError());
-    reduce_partials(epart_.get(), nb_, e_.get(), s);
+    reduce_partials(epart_.get(), nb_, e_out, s);
     for (int a = 0; a < d_; ++a) {
       hipLaunchKernelGGL(k_t3_adj_strength<T>, dim3(rblocks(nk_)), dim3(TB), 0, s, fk_.get(), corr_.get(), k_.get(),
                          d_, a, nk_, uk_.get());
@@ -898,7 +943,6 @@ class NUwNUType3 final : public Potential {
       hipLaunchKernelGGL(k_t3_grad, dim3(rblocks(N_)), dim3(TB), 0, s, x, H_.get(), N_, tm_, a, grad);
       FRESCO_HIP_CHECK(hipGetLastError());
     }
-    return host_.fetch(e_.get(), s);
   }
 
   void get_rho(double* rho, hipStream_t s) override {

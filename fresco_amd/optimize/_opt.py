@@ -129,7 +129,8 @@ class LBFGS_CPP(GradientOptimizer):
     """LBFGS with backtracking line search on the GPU (lbfgs.cpp).
 
     use_graph (not in the reference): replay each iteration as one recorded
-    HIP graph when the potential can be recorded (NUwU, batched NUwU);
+    HIP graph when the potential can be recorded (NUwU, batched NUwU, UwU,
+    UwNU, NUwNU);
     False launches the same kernels one by one, with identical results.
     recursion (not in the reference): "auto" (vector-free from 65536
     unknowns on when M <= 16), "two-loop" (lbfgs.cpp as written) or

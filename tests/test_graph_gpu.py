@@ -56,6 +56,29 @@ def make_case(P, name):
             pats.append((np.ones(N), Sk, V))
             xs.append(rng.uniform(0, 1, 2 * N))
         return P.NUwUBatch(pats, np.ones(2), 1e-6, precision="fp32"), np.concatenate(xs)
+    if name.startswith("uwu"):
+        shape, prec = ((24, 20), "fp64") if name == "uwu2d_fp64" else ((12, 10, 8), "fp32")
+        Sk = rng.uniform(0.5, 3, shape)
+        Sk.flat[::4] = 0.0
+        V = rng.uniform(0, 1, shape)
+        pot = P.UwU(np.array(shape), 0, Sk, V, np.ones(len(shape)), precision=prec)
+        return pot, rng.uniform(0.2, 1.8, int(np.prod(shape)))
+    if name == "uwnu2d_fp32":
+        shape, nk = (16, 12), 150
+        K = rng.uniform(-1.3, 1.3, (nk, 2))
+        Sk = rng.uniform(0.5, 3, nk)
+        V = rng.uniform(0, 1, nk)
+        pot = P.UwNU(np.array(shape), K, Sk, V, np.ones(2), 1e-6, precision="fp32")
+        return pot, rng.uniform(0.2, 1.8, int(np.prod(shape)))
+    if name.startswith("nuwnu"):
+        N, nk = 500, 300
+        K = rng.uniform(-20, 20, (nk, 2))
+        Sk = rng.uniform(0, 2, nk)
+        V = rng.uniform(0, 1, nk)
+        method = "direct" if name == "nuwnu2d_direct" else "nufft"
+        pot = P.NUwNU(rng.uniform(0.5, 1.5, N), K.ravel(), Sk, V, np.ones(2), 1e-6, precision="fp32",
+                      method=method)
+        return pot, rng.uniform(0, 1, 2 * N)
     raise ValueError(name)
 
 
@@ -65,7 +88,8 @@ def same(ra, rb):
     assert ra["H0"] == rb["H0"] and ra.nfev == rb.nfev and ra.nsteps == rb.nsteps
 
 
-@pytest.mark.parametrize("name", ["nuwu2d_fp64", "nuwu2d_fp32", "nuwu3d_fp32", "batch2d_fp32"])
+@pytest.mark.parametrize("name", ["nuwu2d_fp64", "nuwu2d_fp32", "nuwu3d_fp32", "batch2d_fp32", "uwu2d_fp64",
+                                  "uwu3d_fp32", "uwnu2d_fp32", "nuwnu2d_type3", "nuwnu2d_direct"])
 def test_graph_replays_match_launches(mods, name):
     LBFGS_CPP, P = mods
     pot, x0 = make_case(P, name)
@@ -124,8 +148,9 @@ def test_graph_rerecorded_when_targets_grow(mods):
 
 
 def test_graph_off_for_unrecordable_potentials(mods):
-    """Potentials without an enqueue-only evaluation (here a Python one) run
-    the launch-by-launch path with the same results."""
+    """Potentials without an enqueue-only evaluation (here a Python one; also
+    pair and combined potentials) run the launch-by-launch path with the same
+    results."""
     LBFGS_CPP, P = mods
 
     class Quartic:
