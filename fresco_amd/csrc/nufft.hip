@@ -454,9 +454,11 @@ __global__ void __launch_bounds__(64)
   constexpr int SH = Lanes<WMAX>::SH;
   constexpr int NCH = D == 1 ? 1 : (WMAX * WMAX + 63) / 64;
 
-  // batched plans: pattern blockIdx.y has its own bins and grid
+  // batched plans: pattern blockIdx.y has its own bins and grid; sliced
+  // launches: slice blockIdx.z takes the z-th of gridDim.z contiguous parts
+  // of each candidate bin's points and writes its own partial grid
   bin_start += (int64_t)blockIdx.y * a.ntile[0] * a.ntile[1] * a.ntile[2];
-  grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1] * a.nf[2] * NC;
+  grid += ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * a.nf[0] * a.nf[1] * a.nf[2] * NC;
   const int b = blockIdx.x;
   int t[3];
   t[0] = b % a.ntile[0];
@@ -491,8 +493,10 @@ __global__ void __launch_bounds__(64)
         const int bx = cand[t[0] * a.maxc[0] + c0];
         if (bx < 0) continue;
         const int bin = bx + a.ntile[0] * (by + a.ntile[1] * bz);
-        const int beg = bin_start[bin];
-        const int end = bin_start[bin + 1];
+        const int beg0 = bin_start[bin];
+        const int len = bin_start[bin + 1] - beg0;
+        const int beg = beg0 + (int)(((int64_t)len * blockIdx.z) / gridDim.z);
+        const int end = beg0 + (int)(((int64_t)len * (blockIdx.z + 1)) / gridDim.z);
         for (int base = beg; base < end; base += 64) {
           const int j = base + lane;
           uint32_t sc = 0;
@@ -656,7 +660,14 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
   parts = 1;
   if (g.d == 3 && g.tile[0] == 16 && g.tile[1] == 16 && g.tile[2] == 8 && g.nf[1] >= 16 + e.w - 1)
     parts = std::max(1, std::min(16, (8192 + g.nbins - 1) / g.nbins));
-  if (parts > 1) part_buf.allocate((size_t)parts * (size_t)g.ncells());
+  // LDS-tile spreader (1D / 2D) on few tiles: slice every tile's points
+  // over several wavefronts the same way
+  lparts = 1;
+  if (g.d <= 2 && (int64_t)g.nbins * g.batch < 1024)
+    lparts = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + (int64_t)g.nbins * g.batch - 1) /
+                                                                ((int64_t)g.nbins * g.batch)));
+  const size_t pb = std::max((size_t)parts * (size_t)g.ncells(), lparts > 1 ? (size_t)lparts * g.ncells() * g.batch : 0);
+  if (pb > (size_t)g.ncells() || parts > 1) part_buf.allocate(pb);
   // input-driven halo spreader: 3D, w <= 8, 16 x 16 x 8 bins, a halo tile
   // no longer than the grid in any dimension
   halo_ok = g.d == 3 && e.w <= 8 && g.tile[0] == 16 && g.tile[1] == 16 && g.tile[2] == 8 && g.nf[0] >= 23 &&
@@ -751,6 +762,9 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
   halo_parts = pick.parts;
 }
 
+template <typename T>
+__global__ void k_sum_parts(const T* __restrict__ part, int parts, int64_t n, T* __restrict__ grid);
+
 template <typename T, int D, int WMAX>
 static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
   SpreadDev<T> a;
@@ -768,10 +782,20 @@ static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* gr
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
-  ProfScope ps(ST_SPREAD, s);
-  hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins, g.batch), dim3(64), sp.lds_bytes, s, p.scell.get(), p.rec.get(),
-                     p.bin_start.get(), sp.cand.get(), a, grid);
-  FRESCO_HIP_CHECK(hipGetLastError());
+  const int lp = sp.lparts;
+  {
+    ProfScope ps(ST_SPREAD, s);
+    hipLaunchKernelGGL((k_spread<T, D, WMAX>), dim3(g.nbins, g.batch, lp), dim3(64), sp.lds_bytes, s, p.scell.get(),
+                       p.rec.get(), p.bin_start.get(), sp.cand.get(), a, lp > 1 ? sp.part_buf.get() : grid);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+  if (lp > 1) {
+    ProfScope pm(ST_MERGE, s);
+    const size_t n = (size_t)g.ncells() * g.batch;
+    hipLaunchKernelGGL(k_sum_parts<T>, dim3((unsigned)std::min<size_t>(blocks_for(n, 256), 16384)), dim3(256), 0, s,
+                       sp.part_buf.get(), lp, (int64_t)n, grid);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
 }
 
 template <typename T>

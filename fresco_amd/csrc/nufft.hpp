@@ -112,7 +112,8 @@ class Spreader {
   int cand_stride = 0;     // entries per dimension block
   int lds_bytes = 0;
   int parts = 1;             // register-tile spreader: point-range slices per tile
-  DeviceBuffer<T> part_buf;  // [parts][ncells] partial grids when parts > 1
+  int lparts = 1;            // LDS-tile spreader (1D / 2D, real): point-chunk slices per tile
+  DeviceBuffer<T> part_buf;  // [parts][ncells] / [lparts][batch][ncells] partial grids
   // input-driven halo spreader (3D, w <= 8)
   enum Mode { REG = 0, HALO = 1 };
   mutable int mode = REG;

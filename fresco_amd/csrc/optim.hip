@@ -39,6 +39,12 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+__device__ __forceinline__ double wave_sum_xor(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
 // up to three dot products in one pass: (a0.b0, a1.b1, a2.b2)
 __global__ void __launch_bounds__(RB)
     k_dots(const double* __restrict__ a0, const double* __restrict__ b0, const double* __restrict__ a1,
@@ -496,6 +502,131 @@ __global__ void __launch_bounds__(RB)
 }
 
 // ---------------------------------------------------------------------------
+// Small problems (n <= SMALL_T * 16): begin, the slot-by-slot two-loop
+// recursion (lbfgs.cpp:110-160), the uphill test / maxstep factor
+// (lbfgs.cpp:166-186) and the first trial x = xold + factor step in ONE
+// workgroup, the step held in registers (PER values per thread).  Same
+// arithmetic as the multi-kernel recursion except the reduction tree of the
+// dot products; one launch instead of ~6M.
+// ---------------------------------------------------------------------------
+
+constexpr int SMALL_T = 1024;
+constexpr int SMALL_MAXM = 64;
+
+// sum over the workgroup (fixed order); every thread gets the result
+__device__ __forceinline__ double small_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_sum_xor(v);
+  __syncthreads();  // red may still be read by the previous call
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < SMALL_T / 64; ++w) t += red[w];
+  return t;
+}
+
+template <int PER>
+__global__ void __launch_bounds__(SMALL_T)
+    k_lb_small(double* __restrict__ x, const double* __restrict__ g, double* __restrict__ xold,
+               double* __restrict__ gold, double* __restrict__ step, const double* __restrict__ sb,
+               const double* __restrict__ yb, int64_t n, double* __restrict__ dsc, int M) {
+  __shared__ double red[SMALL_T / 64];
+  __shared__ double alpha[SMALL_MAXM];
+  const int tid = threadIdx.x;
+  const int k = (int)dsc[LS_K];
+  const bool first = k == 0;
+  double pref = 1.0;
+  if (first) {
+    double gn = sqrt(dsc[LS_GG]);
+    if (gn > 1.) gn = 1. / gn;
+    pref = -gn * dsc[LS_H0];
+  }
+  double st[PER], gv[PER], xo[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int64_t t = tid + (int64_t)q * SMALL_T;
+    gv[q] = 0.0;
+    xo[q] = 0.0;
+    if (t < n) {
+      gv[q] = g[t];
+      xo[q] = x[t];
+      xold[t] = xo[q];
+      gold[t] = gv[q];
+    }
+    st[q] = first ? pref * gv[q] : gv[q];
+  }
+  const int cnt = k < M ? k : M;
+  for (int j = k - 1; j >= k - cnt; --j) {  // lbfgs.cpp:133-143
+    const int i = j % M;
+    const double* sv = sb + (int64_t)i * n;
+    const double* yv = yb + (int64_t)i * n;
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int64_t t = tid + (int64_t)q * SMALL_T;
+      if (t < n) d += sv[t] * st[q];
+    }
+    const double a = dsc[LS_RHO + i] * small_sum(d, red);
+    if (tid == 0) alpha[i] = a;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int64_t t = tid + (int64_t)q * SMALL_T;
+      if (t < n) st[q] -= a * yv[t];
+    }
+  }
+  if (!first) {  // lbfgs.cpp:144-147
+    const double h = -dsc[LS_H0];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) st[q] *= h;
+  }
+  __syncthreads();  // alpha (LDS) written by thread 0
+  for (int j = k - cnt; j < k; ++j) {  // lbfgs.cpp:149-160
+    const int i = j % M;
+    const double* sv = sb + (int64_t)i * n;
+    const double* yv = yb + (int64_t)i * n;
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int64_t t = tid + (int64_t)q * SMALL_T;
+      if (t < n) d += yv[t] * st[q];
+    }
+    const double c = alpha[i] + dsc[LS_RHO + i] * small_sum(d, red);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int64_t t = tid + (int64_t)q * SMALL_T;
+      if (t < n) st[q] -= c * sv[t];
+    }
+  }
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    d0 += st[q] * gv[q];
+    d1 += st[q] * st[q];
+  }
+  d0 = small_sum(d0, red);
+  d1 = small_sum(d1, red);
+  const double sn = sqrt(d1);
+  double factor = 1.;
+  if (factor * sn > dsc[LS_MAXSTEP]) factor = dsc[LS_MAXSTEP] / sn;
+  const double f = d0 > 0. ? -factor : factor;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int64_t t = tid + (int64_t)q * SMALL_T;
+    if (t < n) {
+      step[t] = st[q];
+      x[t] = xo[q] + f * st[q];
+    }
+  }
+  __syncthreads();  // every thread has read dsc before thread 0 writes it
+  if (tid == 0) {
+    dsc[LS_DOT] = d0;
+    dsc[LS_STEPNORM] = sn;
+    dsc[LS_FACTOR] = f;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Vector-free two-loop recursion (Chen, Wang & Zhou, "Large-scale L-BFGS
 // using MapReduce", NIPS 2014).  The step is a combination of the basis
 // {s_q, y_q (filled slots), g}; the two-loop recursion of lbfgs.cpp:128-160
@@ -859,9 +990,8 @@ void LBFGS::read_scalars() {
 }
 
 // trial x = xold + factor step and its evaluation; the energy lands in
-// dsc_[LS_FNEW].  factor < 0 in `host_factor` means: take LS_FACTOR from the
-// device.  Returns false when the potential has no enqueue-only evaluation
-// and the evaluation ran synchronously instead.
+// dsc_[LS_FNEW].  dev_factor: the first trial's factor, LS_FACTOR on the
+// device; otherwise host_factor (the host's backtracking).
 void LBFGS::enqueue_trial(bool dev_factor, double host_factor, hipStream_t s) {
   const unsigned eb = eblocks(n_);
   if (dev_factor)
@@ -869,6 +999,11 @@ void LBFGS::enqueue_trial(bool dev_factor, double host_factor, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_axpbz, dim3(eb), dim3(256), 0, s, x_.get(), 1.0, xold_.get(), host_factor, step_.get(), n_);
   FRESCO_HIP_CHECK(hipGetLastError());
+  enqueue_eval(s);
+}
+
+// the potential's evaluation at x_ (energy to dsc_[LS_FNEW])
+void LBFGS::enqueue_eval(hipStream_t s) {
   nfev_ += 1;
   if (!pot_->energy_gradient_async(x_.get(), g_.get(), dsc_.get() + LS_FNEW, s)) {
     const double f = pot_->energy_gradient(x_.get(), g_.get(), s);
@@ -965,6 +1100,22 @@ void LBFGS::enqueue_iteration(hipStream_t s) {
     hipLaunchKernelGGL(k_lb_line, dim3(1), dim3(RB), 0, s, part, (int)nb, dsc);
     FRESCO_HIP_CHECK(hipGetLastError());
     enqueue_trial(true, 0.0, s);
+    enqueue_commit(0, s);
+    return;
+  }
+  if (n_ <= 16 * SMALL_T && M_ <= SMALL_MAXM) {
+    const int per = (int)((n_ + SMALL_T - 1) / SMALL_T);
+#define FRESCO_LB_SMALL(P)                                                                                        \
+  hipLaunchKernelGGL(k_lb_small<P>, dim3(1), dim3(SMALL_T), 0, s, x_.get(), g_.get(), xold_.get(), gold_.get(), \
+                     step_.get(), s_.get(), y_.get(), n_, dsc, M_)
+    if (per <= 1) FRESCO_LB_SMALL(1);
+    else if (per <= 2) FRESCO_LB_SMALL(2);
+    else if (per <= 4) FRESCO_LB_SMALL(4);
+    else if (per <= 8) FRESCO_LB_SMALL(8);
+    else FRESCO_LB_SMALL(16);
+#undef FRESCO_LB_SMALL
+    FRESCO_HIP_CHECK(hipGetLastError());
+    enqueue_eval(s);  // the kernel above wrote the trial x
     enqueue_commit(0, s);
     return;
   }

@@ -129,6 +129,7 @@ class LBFGS final : public GradientOptimizer {
   void read_scalars();
   void enqueue_iteration(hipStream_t s);
   void enqueue_trial(bool dev_factor, double host_factor, hipStream_t s);
+  void enqueue_eval(hipStream_t s);
   void enqueue_commit(int force, hipStream_t s);
   void enqueue_gdots(hipStream_t s);
   bool run_graph();
