@@ -822,9 +822,35 @@ static void launch_spread_c(const Spreader<T>& sp, const SortedPoints<T>& p, con
   a.k.two_over_w = (T)(2.0 / sp.es.w);
   const size_t lds = 2 * (size_t)sp.lds_bytes;
   if (lds > 160 * 1024) throw InvalidArgument("complex spreading tile exceeds LDS");
-  hipLaunchKernelGGL((k_spread<T, D, WMAX, 2>), dim3(g.nbins), dim3(64), lds, s, p.scell.get(), p.rec.get(),
-                     p.bin_start.get(), sp.cand.get(), a, grid, str, p.idx.get());
+  const int cp = sp.cparts;
+  hipLaunchKernelGGL((k_spread<T, D, WMAX, 2>), dim3(g.nbins, 1, cp), dim3(64), lds, s, p.scell.get(), p.rec.get(),
+                     p.bin_start.get(), sp.cand.get(), a, cp > 1 ? sp.part_buf_c.get() : grid, str, p.idx.get());
   FRESCO_HIP_CHECK(hipGetLastError());
+  if (cp > 1) {
+    const size_t n = 2 * (size_t)g.ncells();
+    hipLaunchKernelGGL(k_sum_parts<T>, dim3((unsigned)std::min<size_t>(blocks_for(n, 256), 16384)), dim3(256), 0, s,
+                       sp.part_buf_c.get(), cp, (int64_t)n, grid);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+}
+
+template <typename T>
+void Spreader<T>::plan_complex_slices(const SortedPoints<T>& p) {
+  if (geom.batch != 1) return;
+  std::vector<int> st((size_t)geom.nbins + 1);
+  FRESCO_HIP_CHECK(hipMemcpy(st.data(), p.bin_start.get(), st.size() * sizeof(int), hipMemcpyDeviceToHost));
+  int busy = 0, mx = 0;
+  for (int b = 0; b < geom.nbins; ++b) {
+    const int c = st[b + 1] - st[b];
+    busy += c > 0;
+    mx = std::max(mx, c);
+  }
+  // about 2048 wavefronts over the tiles that hold points, slices of >= 32
+  // points
+  int cp = busy > 0 ? (2048 + busy - 1) / busy : 1;
+  cp = std::max(1, std::min({cp, 16, mx / 32}));
+  if (cp > 1) part_buf_c.allocate((size_t)cp * 2 * (size_t)geom.ncells());
+  cparts = cp;
 }
 
 template <typename T>

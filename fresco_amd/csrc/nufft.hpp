@@ -104,6 +104,10 @@ class Spreader {
   void spread_complex(const SortedPoints<T>& p, const Cplx<T>* str, Cplx<T>* grid, hipStream_t s) const;
   // time the 3D spreader configurations on these points, keep the fastest
   void autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
+  // complex spreading of a point set that stays fixed (the k points of
+  // UwNU / NUwNU): point-range slices per tile chosen from its bin counts
+  // (one host read, at set-up)
+  void plan_complex_slices(const SortedPoints<T>& p);
 
   GridGeom geom;
   EsParams es;
@@ -112,7 +116,9 @@ class Spreader {
   int cand_stride = 0;     // entries per dimension block
   int lds_bytes = 0;
   int parts = 1;             // register-tile spreader: point-range slices per tile
-  int lparts = 1;            // LDS-tile spreader (1D / 2D, real): point-chunk slices per tile
+  int lparts = 1;            // LDS-tile spreader (1D / 2D, real): point-range slices per tile
+  int cparts = 1;            // the same for complex spreading (plan_complex_slices)
+  DeviceBuffer<T> part_buf_c;  // [cparts][ncells][2]
   DeviceBuffer<T> part_buf;  // [parts][ncells] / [lparts][batch][ncells] partial grids
   // input-driven halo spreader (3D, w <= 8)
   enum Mode { REG = 0, HALO = 1 };

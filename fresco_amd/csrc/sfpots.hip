@@ -319,6 +319,7 @@ class UwNU final : public Potential {
     cm.wrap = 0;
     pts_.set(kd.get(), cm, nullptr, nullptr);
     FRESCO_HIP_CHECK(hipDeviceSynchronize());
+    spreader_.plan_complex_slices(pts_);
     fine_.allocate(geom_.ncells());
     fine2_.allocate(geom_.ncells());
     const bool dbl = sizeof(T) == 8;
@@ -848,6 +849,7 @@ class NUwNUType3 final : public Potential {
     tmap.wrap = 0;
     tgt_.set(sd.get(), tmap, nullptr, nullptr);
     FRESCO_HIP_CHECK(hipDeviceSynchronize());
+    sp2_.plan_complex_slices(tgt_);
     corr_.allocate(corr.size());
     corr_.upload(corr.data(), corr.size());
     k_.allocate((size_t)nk_ * d);
