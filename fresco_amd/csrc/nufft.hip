@@ -160,9 +160,12 @@ GridGeom make_geom(int d, const int* nf, int w, int elem_bytes, int batch) {
   for (int a = 0; a < 3; ++a) g.nf[a] = a < d ? nf[a] : 1;
   for (int a = 0; a < 3; ++a) g.tile[a] = 1;
   if (d == 3) {
-    // the register-tile spreader's layout (16 x 16 x 8); not shrunk further
+    // 16 x 16 x 8: the register-tile spreader's layout; FP32 grids with a
+    // kernel up to 8 wide use 16 x 8 x 8 bins for the halo spreader (23 KB
+    // halo tiles: more wavefronts per CU, no point slicing needed, half the
+    // halo bytes of 4-sliced 16 x 16 x 8 bins).  Not shrunk further.
     g.tile[0] = 16;
-    g.tile[1] = 16;
+    g.tile[1] = (elem_bytes == 4 && w <= 8) ? 8 : 16;
     g.tile[2] = 8;
   } else if (d == 2) {
     g.tile[0] = g.tile[1] = 64;
@@ -401,15 +404,20 @@ __device__ __forceinline__ T wave_sum63(T v) {
 // spreading: one wavefront owns one tile (LDS), owner-computes
 // ---------------------------------------------------------------------------
 
-// halo tile of the input-driven 3D spreader (k_spread3_halo): 16 x 16 x 8
-// bins plus the one-sided halo of a kernel up to 8 wide
-constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8, HW = 8;
-constexpr int HP_X = HT_X + HW - 1, HP_Y = HT_Y + HW - 1, HP_Z = HT_Z + HW - 1;
+// halo tile of the input-driven 3D spreader (k_spread3_halo): 16 x TY x 8
+// bins (TY = 16, or 8 for the FP32 grids) plus the one-sided halo of a kernel
+// up to 8 wide
+constexpr int HW = 8;
 constexpr int HS_X = 24;  // row stride: (ja, jb) lane pattern conflict-free for 4- and 8-byte cells
-// plane stride in rows: HS_X * HS_Y cells is a multiple of 64 words, so the 8
-// z-cells of a lane pair up into read2st64 / write2st64 LDS instructions
-constexpr int HS_Y = 24;
-constexpr int HT_ELEMS = HS_X * HS_Y * HP_Z;
+constexpr int HP_Z = 8 + HW - 1;
+template <int TY>
+struct HaloShape {
+  static constexpr int HP_Y = TY + HW - 1;
+  // plane stride in rows: HS_X * HS_Y cells is a multiple of 64 words, so
+  // the 8 z-cells of a lane pair up into read2st64 / write2st64 instructions
+  static constexpr int HS_Y = TY == 16 ? 24 : 16;
+  static constexpr int ELEMS = HS_X * HS_Y * HP_Z;
+};
 
 template <typename T>
 struct SpreadDev {
@@ -670,13 +678,13 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
   if (pb > (size_t)g.ncells() || parts > 1) part_buf.allocate(pb);
   // input-driven halo spreader: 3D, w <= 8, 16 x 16 x 8 bins, a halo tile
   // no longer than the grid in any dimension
-  halo_ok = g.d == 3 && e.w <= 8 && g.tile[0] == 16 && g.tile[1] == 16 && g.tile[2] == 8 && g.nf[0] >= 23 &&
-            g.nf[1] >= 23 && g.nf[2] >= 15;
+  halo_ok = g.d == 3 && e.w <= 8 && g.tile[0] == 16 && (g.tile[1] == 16 || g.tile[1] == 8) && g.tile[2] == 8 &&
+            g.nf[0] >= 23 && g.nf[1] >= g.tile[1] + 7 && g.nf[2] >= 15;
   if (halo_ok) {
     std::vector<int> tabs;
     for (int a = 0; a < 3; ++a) {
       std::vector<int> t(4 * (size_t)g.nf[a], -1);
-      const int reach[3] = {16 + 7, 16 + 7, 8 + 7};
+      const int reach[3] = {g.tile[0] + 7, g.tile[1] + 7, g.tile[2] + 7};
       for (int b = 0; b < g.ntile[a]; ++b) {
         const int t0 = b * g.tile[a];
         const int tl = std::min(g.tile[a], g.nf[a] - t0);
@@ -696,7 +704,8 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     // slices per bin: more slices, more wavefronts in flight but more halo
     // tiles to write and merge; the autotuner picks among these
     max_halo_parts = std::max(1, std::min(16, (16384 + g.nbins - 1) / g.nbins));
-    halo_buf.allocate((size_t)g.nbins * max_halo_parts * HT_ELEMS);
+    const int helems = g.tile[1] == 8 ? HaloShape<8>::ELEMS : HaloShape<16>::ELEMS;
+    halo_buf.allocate((size_t)g.nbins * max_halo_parts * helems);
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
@@ -724,7 +733,7 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
     int mode, parts;
   };
   std::vector<Cand> cands;
-  cands.push_back({REG, 1});
+  if (geom.tile[0] == 16 && geom.tile[1] == 16 && geom.tile[2] == 8) cands.push_back({REG, 1});
   for (int sl : {1, 2, 3, 4, 6, 8, 12, 16})
     if (sl <= max_halo_parts) cands.push_back({HALO, sl});
   hipEvent_t e0, e1;
@@ -1316,10 +1325,12 @@ __global__ void __launch_bounds__(LANE_INTERP_THREADS)
 // ---------------------------------------------------------------------------
 
 
-template <typename T>
+template <typename T, int TY>
 __global__ void __launch_bounds__(64)
     k_spread3_halo(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
                    const int* __restrict__ bin_start, SpreadDev<T> a, T* __restrict__ halo) {
+  constexpr int HS_Y = HaloShape<TY>::HS_Y;
+  constexpr int HT_ELEMS = HaloShape<TY>::ELEMS;
   __shared__ __align__(16) T tile[HT_ELEMS];
   const int lane = threadIdx.x;
   const int w = a.w;
@@ -1395,9 +1406,11 @@ __global__ void __launch_bounds__(64)
 
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
 // covering fine-grid coordinate c along dimension a, -1 terminated
-template <typename T>
+template <typename T, int TY>
 __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__ tabs, int parts, SpreadDev<T> a,
                              T* __restrict__ grid) {
+  constexpr int HS_Y = HaloShape<TY>::HS_Y;
+  constexpr int HT_ELEMS = HaloShape<TY>::ELEMS;
   const int64_t n = (int64_t)a.nf[0] * a.nf[1] * a.nf[2];
   const int* tx = tabs;
   const int* ty = tabs + 4 * a.nf[0];
@@ -1444,17 +1457,26 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   a.k.beta = (T)sp.es.beta;
   a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
   a.k.two_over_w = (T)(2.0 / sp.es.w);
+  const bool t8 = g.tile[1] == 8;
   {
     ProfScope ps(ST_SPREAD, s);
-    hipLaunchKernelGGL(k_spread3_halo<T>, dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(), p.rec.get(),
-                       p.bin_start.get(), a, sp.halo_buf.get());
+    if (t8)
+      hipLaunchKernelGGL((k_spread3_halo<T, 8>), dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(),
+                         p.rec.get(), p.bin_start.get(), a, sp.halo_buf.get());
+    else
+      hipLaunchKernelGGL((k_spread3_halo<T, 16>), dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(),
+                         p.rec.get(), p.bin_start.get(), a, sp.halo_buf.get());
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);
   const size_t n = (size_t)g.ncells();
   const dim3 grd((unsigned)std::min<size_t>(blocks_for(n, 256), 16384));
-  hipLaunchKernelGGL(k_merge_halo<T>, grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), sp.halo_parts, a,
-                     grid);
+  if (t8)
+    hipLaunchKernelGGL((k_merge_halo<T, 8>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(),
+                       sp.halo_parts, a, grid);
+  else
+    hipLaunchKernelGGL((k_merge_halo<T, 16>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(),
+                       sp.halo_parts, a, grid);
   FRESCO_HIP_CHECK(hipGetLastError());
   return true;
 }
