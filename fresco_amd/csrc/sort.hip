@@ -1,9 +1,9 @@
 // Stable LSD radix sort of (bin key, point index) pairs with digits of up to
 // 11 bits (keys below 2^11, such as the bins of the 3D band grids, take one
-// pass).
+// pass), 4096-key tiles.
 //
 // Per digit pass:
-//   k_rs_upsweep   digit counts of every 1024-key tile (LDS histogram)
+//   k_rs_upsweep   digit counts of every 4096-key tile (LDS histogram)
 //   k_rs_scan_seg  exclusive scan of the digit-major count table in
 //                  segments of 4096 entries (one workgroup each, coalesced)
 //   k_rs_scan_top  exclusive scan of the segment totals (one workgroup)
@@ -28,7 +28,7 @@ namespace fresco_hip {
 namespace {
 
 constexpr int RS_T = 256;                  // threads per tile workgroup
-constexpr int RS_ROUNDS = 4;               // rounds of 256 keys per tile
+constexpr int RS_ROUNDS = 16;              // rounds of 256 keys per tile
 constexpr int RS_TILE = RS_T * RS_ROUNDS;  // keys per tile
 constexpr int RS_MAXB = 2048;              // buckets of an 11-bit digit
 constexpr int RS_SEG = 4096;               // scan segment
@@ -124,14 +124,14 @@ __global__ void __launch_bounds__(RS_T)
     const int64_t i = (int64_t)d * nblk + blockIdx.x;
     run[d] = offs[i] + tops[i / RS_SEG];
   }
+  for (int d = t; d < nbuck; d += RS_T) {
+#pragma unroll
+    for (int w = 0; w < RS_T / 64; ++w) wc[w][d] = 0;
+  }
+  __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   for (int r = 0; r < RS_ROUNDS; ++r) {
-    for (int d = t; d < nbuck; d += RS_T) {
-#pragma unroll
-      for (int w = 0; w < RS_T / 64; ++w) wc[w][d] = 0;
-    }
-    __syncthreads();
     const int64_t e = base + (int64_t)r * RS_T + t;
     const bool valid = e < n;
     const uint32_t k = valid ? kin[e] : 0u;
@@ -157,11 +157,12 @@ __global__ void __launch_bounds__(RS_T)
       }
     }
     __syncthreads();
-    for (int d = t; d < nbuck; d += RS_T) {
-      uint32_t add = 0;
-#pragma unroll
-      for (int w = 0; w < RS_T / 64; ++w) add += wc[w][d];
-      run[d] += add;
+    // only the digits of this round move: their leaders advance the running
+    // offsets (integer LDS adds: the order does not matter) and clear their
+    // counts, so a round costs O(keys), not O(buckets)
+    if (valid && rank == 0) {
+      atomicAdd(&run[dg], wc[wv][dg]);
+      wc[wv][dg] = 0;
     }
     __syncthreads();
   }
