@@ -194,8 +194,10 @@ GridGeom make_geom(int d, const int* nf, int w, int elem_bytes, int batch) {
     // spreading tile <= 40 KB; enough tiles to fill 256 CUs
     while (interp_lds_bytes(g, elem_bytes, w) > 80 * 1024 || spread_lds_bytes(g, elem_bytes, w) > 40 * 1024)
       if (!halve_largest(4)) break;
+    // enough tiles per pattern that the largest pattern of a batch (whose
+    // tiles take the longest) spreads over the GPU by itself
     const int minsz = d == 2 ? 16 : 64;
-    while (ntiles() * batch < 1024)
+    while (ntiles() < 1024)
       if (!halve_largest(minsz)) break;
     clamp();
   }
@@ -1778,6 +1780,144 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// 2D interpolation, eight lanes per point (w <= 8): the bin's (tile + 7)^2
+// grid region is staged in LDS; lane ja of an 8-lane group sums the 8 cells
+// of its x offset with the y weights (and their derivatives) shared by
+// ds_swizzle, multiplies by its x weight (or its derivative), and the group
+// reduces with three DPP steps.  blockIdx.y: pattern of a batched plan;
+// blockIdx.z: contiguous slice of the bin's points.
+// ---------------------------------------------------------------------------
+
+template <typename T, bool DERIV>
+__global__ void __launch_bounds__(G8_THREADS)
+    k_interp2_g8(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                 const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
+                 InterpDev<T> a, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* reg = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  bin_start += (int64_t)blockIdx.y * a.ntile[0] * a.ntile[1];
+  grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1];
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.z) / gridDim.z);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.z + 1)) / gridDim.z);
+  if (beg == end) return;  // uniform across the workgroup
+  const int t0x = (b % a.ntile[0]) * a.tile[0];
+  const int t0y = (b / a.ntile[0]) * a.tile[1];
+  const int tot = a.rx * a.ry;
+  for (int e = tid; e < tot; e += G8_THREADS) {
+    const int lx = e % a.rx;
+    const int ly = e / a.rx;
+    const int gx = (t0x + lx) % a.nf[0];  // the region may wrap
+    const int gy = (t0y + ly) % a.nf[1];
+    reg[ly * a.srx + lx] = grid[(int64_t)gy * a.nf[0] + gx];
+  }
+  __syncthreads();
+  const int ja = tid & 7;
+  const int grp = tid >> 3;
+  const int npts = end - beg;
+  const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
+  int kn = beg + grp;
+  uint32_t sc_n = scell[kn < end ? kn : beg];
+  typename Vec4<T>::type r_n = rec[kn < end ? kn : beg];
+  uint32_t j_n = idx[kn < end ? kn : beg];
+  for (int it = 0; it < iters; ++it) {
+    const int k = kn;
+    const bool valid = k < end;
+    const uint32_t sc = sc_n;
+    const typename Vec4<T>::type r = r_n;
+    const uint32_t jout = j_n;
+    kn = k + G8_THREADS / 8;
+    if (it + 1 < iters) {
+      const int kq = kn < end ? kn : beg;
+      sc_n = scell[kq];
+      r_n = rec[kq];
+      j_n = idx[kq];
+    }
+    int ux = (int)(sc & cmask) - t0x;
+    if (ux < 0) ux += a.nf[0];
+    int uy = (int)((sc >> a.cbits) & cmask) - t0y;
+    if (uy < 0) uy += a.nf[1];
+    T wxa = T(0), dwxa = T(0), wya = T(0), dwya = T(0);
+    if (ja < w) {
+      if (DERIV) {
+        es_phi_dphi<T>((T(ja) - r.x) * a.k.two_over_w, a.k, wxa, dwxa);
+        es_phi_dphi<T>((T(ja) - r.y) * a.k.two_over_w, a.k, wya, dwya);
+      } else {
+        wxa = es_phi<T>((T(ja) - r.x) * a.k.two_over_w, a.k);
+        wya = es_phi<T>((T(ja) - r.y) * a.k.two_over_w, a.k);
+      }
+    }
+    T wy[8], dwy[8];
+    group8_bcast<T>(wya, wy);
+    if (DERIV) group8_bcast<T>(dwya, dwy);
+    // all 8 x 8 cells are read (weights beyond w are zero and the region is
+    // staged 8 - 1 cells wider than the tile), so the loads can be batched
+    const T* col = reg + uy * a.srx + ux + ja;
+    T sA = T(0), sB = T(0);
+#pragma unroll
+    for (int jb = 0; jb < 8; ++jb) {
+      const T g = col[jb * a.srx];
+      sA += g * wy[jb];
+      if (DERIV) sB += g * dwy[jb];
+    }
+    T vx, vy = T(0);
+    if (DERIV) {
+      vx = sum8_last(dwxa * sA);
+      vy = sum8_last(wxa * sB);
+    } else {
+      vx = sum8_last(wxa * sA);
+    }
+    if (ja == 7 && valid) {
+      const int64_t j = jout;
+      const double cw = (double)r.w;
+      if (DERIV) {
+        out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
+        out[j * a.ostride + 1] = cw * a.dscale[1] * (double)vy;
+      } else {
+        out[j * a.ostride + a.ocomp] = cw * (double)vx;
+      }
+    }
+  }
+}
+
+template <typename T, bool DERIV>
+static bool launch_interp2_g8(const SortedPoints<T>& p, const T* grid, const double* dscale, double* out,
+                              int ostride, int ocomp, hipStream_t s) {
+  InterpDev<T> a;
+  const GridGeom& g = p.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.dscale[q] = dscale ? dscale[q] : 0.0;
+  }
+  a.cbits = g.cell_bits;
+  a.w = p.es.w;
+  a.rx = g.tile[0] + 8 - 1;
+  a.ry = g.tile[1] + 8 - 1;
+  a.rz = 1;
+  a.srx = a.rx;
+  a.ostride = ostride;
+  a.ocomp = ocomp;
+  a.k.beta = (T)p.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
+  a.k.two_over_w = (T)(2.0 / p.es.w);
+  const size_t lds = sizeof(T) * (size_t)a.rx * a.ry;
+  if (lds > 64 * 1024) return false;
+  const int64_t wg = (int64_t)g.nbins * g.batch;
+  const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(8, (2048 + wg - 1) / wg));
+  hipLaunchKernelGGL((k_interp2_g8<T, DERIV>), dim3(g.nbins, g.batch, chunks), dim3(G8_THREADS), lds, s,
+                     p.scell.get(), p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 template <typename T, int D, int WMAX>
 static void launch_interp(const SortedPoints<T>& p, const T* grid, bool deriv, const double* dscale, double* out,
                           int ostride, int ocomp, hipStream_t s) {
@@ -1835,6 +1975,9 @@ void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const doub
       launch_interp<T, 3, 16>(p, grid, deriv, dscale, out, ostride, ocomp, s);
     }
   } else if (d == 2) {
+    if (small && (deriv ? launch_interp2_g8<T, true>(p, grid, dscale, out, ostride, ocomp, s)
+                        : launch_interp2_g8<T, false>(p, grid, dscale, out, ostride, ocomp, s)))
+      return;
     small ? launch_interp<T, 2, 8>(p, grid, deriv, dscale, out, ostride, ocomp, s)
           : launch_interp<T, 2, 16>(p, grid, deriv, dscale, out, ostride, ocomp, s);
   } else {
