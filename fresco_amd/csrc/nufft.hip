@@ -682,6 +682,39 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
   // no longer than the grid in any dimension
   halo_ok = g.d == 3 && e.w <= 8 && g.tile[0] == 16 && (g.tile[1] == 16 || g.tile[1] == 8) && g.tile[2] == 8 &&
             g.nf[0] >= 23 && g.nf[1] >= g.tile[1] + 7 && g.nf[2] >= 15;
+  // 2D input-driven halo spreader (real strengths, w <= 8): rows of
+  // tile + 7 cells, stride a multiple of 8 with an odd factor
+  halo2_ok = g.d == 2 && e.w <= 8 && g.tile[0] + 7 <= g.nf[0] && g.tile[1] + 7 <= g.nf[1] && g.tile[0] + 7 < 256 &&
+             g.tile[1] + 7 < 256;
+  if (halo2_ok) {
+    int hsx = ((g.tile[0] + 7 + 7) / 8) * 8;
+    if ((hsx / 8) % 2 == 0) hsx += 8;
+    halo2_hs = hsx;
+    halo2_rows = g.tile[1] + 7;
+    std::vector<int> tabs;
+    for (int a = 0; a < 2; ++a) {
+      std::vector<int> t(4 * (size_t)g.nf[a], -1);
+      const int reach = g.tile[a] + 7;
+      for (int b = 0; b < g.ntile[a]; ++b) {
+        const int t0 = b * g.tile[a];
+        const int tl = std::min(g.tile[a], g.nf[a] - t0);
+        const int ulim = std::min(reach, tl + e.w - 1);
+        for (int u = 0; u < ulim; ++u) {
+          const int c = (t0 + u) % g.nf[a];
+          int k = 0;
+          while (k < 4 && t[4 * c + k] >= 0) ++k;
+          if (k == 4) throw RuntimeFailure("halo merge table overflow");
+          t[4 * c + k] = (b << 8) | u;
+        }
+      }
+      tabs.insert(tabs.end(), t.begin(), t.end());
+    }
+    halo_tab.allocate(tabs.size());
+    halo_tab.upload(tabs.data(), tabs.size());
+    const int64_t wg = (int64_t)g.nbins * g.batch;
+    halo2_parts = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + wg - 1) / wg));
+    halo_buf.allocate((size_t)wg * halo2_parts * (size_t)halo2_hs * halo2_rows);
+  }
   if (halo_ok) {
     std::vector<int> tabs;
     for (int a = 0; a < 3; ++a) {
@@ -888,6 +921,7 @@ void Spreader<T>::spread(const SortedPoints<T>& p, T* grid, hipStream_t s) const
     if (launch_spread3_reg<T>(*this, p, grid, s)) return;
     small ? launch_spread<T, 3, 8>(*this, p, grid, s) : launch_spread<T, 3, 16>(*this, p, grid, s);
   } else if (d == 2) {
+    if (launch_spread2_halo<T>(*this, p, grid, s)) return;
     small ? launch_spread<T, 2, 8>(*this, p, grid, s) : launch_spread<T, 2, 16>(*this, p, grid, s);
   } else {
     launch_spread<T, 1, 16>(*this, p, grid, s);
@@ -1439,6 +1473,143 @@ __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__
     }
     grid[i] = acc;
   }
+}
+
+// ---------------------------------------------------------------------------
+// 2D input-driven spreading (w <= 8, real strengths): a bin's supports lie in
+// its (tile + 7)^2 halo tile; one wavefront per (bin, slice, pattern) adds its
+// points there in LDS (lane (ja, jb) = cell (x + ja, y + jb) of the support;
+// rows of stride hs, a multiple of 8 with an odd factor so the lane pattern is
+// bank-conflict-free), two points per iteration sharing the kernel
+// evaluation; every point is handled once.  k_merge_halo2 then sums, per
+// fine cell, the halo tiles covering it in a fixed order (reproducible).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_spread2_halo(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                   const int* __restrict__ bin_start, SpreadDev<T> a, int hs, int hrows, T* __restrict__ halo) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int helems = hs * hrows;
+  const int lane = threadIdx.x;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int nbins = a.ntile[0] * a.ntile[1];
+  bin_start += (int64_t)blockIdx.y * nbins;
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.z) / gridDim.z);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.z + 1)) / gridDim.z);
+  T* out = halo + (((int64_t)blockIdx.y * nbins + b) * gridDim.z + blockIdx.z) * helems;
+  for (int e = lane; e < helems; e += 64) tile[e] = T(0);
+  if (beg == end) {
+    for (int e = lane; e < helems; e += 64) out[e] = T(0);
+    return;
+  }
+  const int t0x = (b % a.ntile[0]) * a.tile[0];
+  const int t0y = (b / a.ntile[0]) * a.tile[1];
+  __syncthreads();
+  const int hb = lane >> 5;         // kernel-value half: 0 point A, 1 point B
+  const int kd = (lane & 31) >> 3;  // dimension of this lane's kernel value
+  const int kj = lane & 7;
+  const bool kact = kd < 2 && kj < w;
+  const int ja = lane & 7, jb = lane >> 3;
+  for (int base = beg; base < end; base += 64) {
+    const int cnt = min(64, end - base);
+    uint32_t sc = 0;
+    typename Vec4<T>::type rr;
+    rr.x = rr.y = rr.z = rr.w = T(0);
+    if (lane < cnt) {
+      sc = scell[base + lane];
+      rr = rec[base + lane];
+    }
+    for (int p = 0; p < cnt; p += 2) {
+      const int q = p + 1 < cnt ? p + 1 : p;  // odd tail: B repeats A with weight 0
+      const uint32_t sa = lane_read_u(sc, p), sb = lane_read_u(sc, q);
+      const T cA = lane_read(rr.w, p);
+      const T cB = p + 1 < cnt ? lane_read(rr.w, q) : T(0);
+      const T oAx = lane_read(rr.x, p), oAy = lane_read(rr.y, p);
+      const T oBx = lane_read(rr.x, q), oBy = lane_read(rr.y, q);
+      const T off = hb ? (kd == 0 ? oBx : oBy) : (kd == 0 ? oAx : oAy);
+      const T kv = kact ? es_phi<T>((T(kj) - off) * a.k.two_over_w, a.k) : T(0);
+      const T vA = lane_get(kv, ja) * lane_get(kv, 8 + jb) * cA;
+      const T vB = lane_get(kv, 32 + ja) * lane_get(kv, 40 + jb) * cB;
+      int uAx = (int)(sa & cmask) - t0x, uAy = (int)((sa >> a.cbits) & cmask) - t0y;
+      int uBx = (int)(sb & cmask) - t0x, uBy = (int)((sb >> a.cbits) & cmask) - t0y;
+      tile[(uAy + jb) * hs + uAx + ja] += vA;
+      tile[(uBy + jb) * hs + uBx + ja] += vB;
+    }
+  }
+  __syncthreads();
+  for (int e = lane; e < helems; e += 64) out[e] = tile[e];
+}
+
+// per fine cell of every pattern: the sum of the halo tiles covering it
+// (tabs as k_merge_halo's, x then y), slices in order
+template <typename T>
+__global__ void k_merge_halo2(const T* __restrict__ halo, const int* __restrict__ tabs, int parts, int hs,
+                              int helems, SpreadDev<T> a, int batch, T* __restrict__ grid) {
+  const int64_t nc = (int64_t)a.nf[0] * a.nf[1];
+  const int64_t n = nc * batch;
+  const int nbins = a.ntile[0] * a.ntile[1];
+  const int* tx = tabs;
+  const int* ty = tabs + 4 * a.nf[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pat = i / nc;
+    const int64_t c = i - pat * nc;
+    const int gx = (int)(c % a.nf[0]);
+    const int gy = (int)(c / a.nf[0]);
+    T acc = T(0);
+    for (int iy = 0; iy < 4; ++iy) {
+      const int ey = ty[gy * 4 + iy];
+      if (ey < 0) break;
+      for (int ix = 0; ix < 4; ++ix) {
+        const int ex = tx[gx * 4 + ix];
+        if (ex < 0) break;
+        const int64_t bin = pat * nbins + (ex >> 8) + (int64_t)a.ntile[0] * (ey >> 8);
+        const int64_t off = (int64_t)(ey & 255) * hs + (ex & 255);
+        for (int sl = 0; sl < parts; ++sl) acc += halo[(bin * parts + sl) * helems + off];
+      }
+    }
+    grid[i] = acc;
+  }
+}
+
+template <typename T>
+static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
+  if (!sp.halo2_ok) return false;
+  SpreadDev<T> a;
+  const GridGeom& g = sp.geom;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.maxc[q] = sp.maxc[q];
+  }
+  a.cbits = g.cell_bits;
+  a.w = sp.es.w;
+  a.sx = sp.halo2_hs;
+  a.cstride = sp.cand_stride;
+  a.k.beta = (T)sp.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(sp.es.beta * M_LOG2E) : (T)sp.es.beta;
+  a.k.two_over_w = (T)(2.0 / sp.es.w);
+  const int hs = sp.halo2_hs, hrows = sp.halo2_rows;
+  const int helems = hs * hrows;
+  {
+    ProfScope ps(ST_SPREAD, s);
+    hipLaunchKernelGGL(k_spread2_halo<T>, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64),
+                       sizeof(T) * (size_t)helems, s, p.scell.get(), p.rec.get(), p.bin_start.get(), a, hs, hrows,
+                       sp.halo_buf.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+  }
+  ProfScope pm(ST_MERGE, s);
+  const size_t n = (size_t)g.ncells() * g.batch;
+  hipLaunchKernelGGL(k_merge_halo2<T>, dim3((unsigned)std::min<size_t>(blocks_for(n, 256), 16384)), dim3(256), 0, s,
+                     sp.halo_buf.get(), sp.halo_tab.get(), sp.halo2_parts, hs, helems, a, g.batch, grid);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 template <typename T>

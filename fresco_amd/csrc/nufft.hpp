@@ -129,6 +129,10 @@ class Spreader {
   int max_halo_parts = 1;
   DeviceBuffer<T> halo_buf;    // [nbins][halo_parts][halo tile]
   DeviceBuffer<int> halo_tab;  // per-dimension coverage tables for the merge
+  // input-driven 2D spreader (real strengths, w <= 8)
+  bool halo2_ok = false;
+  int halo2_hs = 0, halo2_rows = 0;  // halo tile row stride and rows
+  int halo2_parts = 1;               // point-range slices per bin
 };
 
 // Interpolation of a real fine grid at the sorted points.
