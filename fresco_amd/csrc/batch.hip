@@ -179,6 +179,7 @@ class BatchNUwU final : public Potential {
     for (int a = 0; a < d; ++a) nf[a] = next_smooth_even((int)std::ceil(sigma * (2 * band + 1)), es_.w);
     geom_ = make_geom(d, nf, es_.w, (int)sizeof(T), npat);
     pts_.init(geom_, es_, Ntot_);
+    pts_.set_segments(pt0_);
     spreader_.init(geom_, es_);
     G_ = geom_.ncells();
     grid_.allocate((size_t)G_ * npat);

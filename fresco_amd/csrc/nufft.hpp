@@ -77,6 +77,10 @@ class SortedPoints {
  public:
   using R4 = typename Vec4<T>::type;
   void init(const GridGeom& g, const EsParams& es, int64_t n);
+  // batched plans whose points are stored pattern by pattern: pattern p owns
+  // points [starts[p], starts[p+1]); the bin sort then runs one pass per
+  // pattern instead of sorting the combined keys
+  void set_segments(const std::vector<int64_t>& starts) { sorter.set_segments(starts, geom.nbins); }
   // x: device float64 [n*d]; c: device T[n] weights (may be null -> 1);
   // pat: device pattern index of every point (geom.batch > 1), else null.
   void set(const double* x, const CoordMap& m, const T* c, hipStream_t s, const uint32_t* pat = nullptr);

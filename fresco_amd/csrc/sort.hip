@@ -168,6 +168,106 @@ __global__ void __launch_bounds__(RS_T)
   }
 }
 
+// ---------------------------------------------------------------------------
+// segmented single pass: tiles never straddle segments; counts are
+// tile-major cnt[t * nbuck + d]
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(RS_T)
+    k_rs_seg_upsweep(const uint32_t* __restrict__ keys, const int64_t* __restrict__ tb,
+                     const int64_t* __restrict__ te, uint32_t nbuck, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[RS_MAXB];
+  const int t = threadIdx.x;
+  for (int d = t; d < (int)nbuck; d += RS_T) h[d] = 0;
+  __syncthreads();
+  const int64_t beg = tb[blockIdx.x], end = te[blockIdx.x];
+  for (int64_t e = beg + t; e < end; e += RS_T) atomicAdd(&h[keys[e] % nbuck], 1u);
+  __syncthreads();
+  for (int d = t; d < (int)nbuck; d += RS_T) cnt[(int64_t)blockIdx.x * nbuck + d] = h[d];
+}
+
+// one workgroup per segment: exclusive offsets over (digit, tile) in
+// digit-major order, plus the segment's start
+__global__ void __launch_bounds__(RS_T)
+    k_rs_seg_scan(uint32_t* __restrict__ cnt, const int* __restrict__ tile0, const int64_t* __restrict__ sstart,
+                  int nbuck) {
+  __shared__ uint32_t sh[RS_T];
+  const int sg = blockIdx.x;
+  const int t0 = tile0[sg], nt = tile0[sg + 1] - t0;
+  const int64_t m = (int64_t)nt * nbuck;  // entries, digit-major: e = d * nt + (t - t0)
+  const int64_t per = (m + RS_T - 1) / RS_T;
+  const int64_t lo = threadIdx.x * per < m ? threadIdx.x * per : m;
+  const int64_t hi = lo + per < m ? lo + per : m;
+  uint32_t sum = 0;
+  for (int64_t e = lo; e < hi; ++e) {
+    const int64_t d = e / nt, tt = e % nt;
+    sum += cnt[(t0 + tt) * nbuck + d];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, sh, total) + (uint32_t)sstart[sg];
+  for (int64_t e = lo; e < hi; ++e) {
+    const int64_t d = e / nt, tt = e % nt;
+    const int64_t i = (t0 + tt) * nbuck + d;
+    const uint32_t c = cnt[i];
+    cnt[i] = run;
+    run += c;
+  }
+}
+
+// k_rs_scatter for a segment tile: running offsets from the tile-major table
+__global__ void __launch_bounds__(RS_T)
+    k_rs_seg_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                     const int64_t* __restrict__ tb, const int64_t* __restrict__ te, int dbits, uint32_t nbuck,
+                     const uint32_t* __restrict__ offs, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                     const uint4* __restrict__ pin, uint4* __restrict__ pout, int pq, const uint32_t* __restrict__ cin,
+                     uint32_t* __restrict__ cout) {
+  __shared__ uint32_t run[RS_MAXB];
+  __shared__ uint32_t wc[RS_T / 64][RS_MAXB];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  for (int d = t; d < (int)nbuck; d += RS_T) {
+    run[d] = offs[(int64_t)blockIdx.x * nbuck + d];
+#pragma unroll
+    for (int w = 0; w < RS_T / 64; ++w) wc[w][d] = 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int64_t beg = tb[blockIdx.x], end = te[blockIdx.x];
+  for (int64_t r0 = beg; r0 < end; r0 += RS_T) {
+    const int64_t e = r0 + t;
+    const bool valid = e < end;
+    const uint32_t k = valid ? kin[e] : 0u;
+    const uint32_t dg = k % nbuck;
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+      const bool bit = (dg >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const int rank = __popcll(same & lt);
+    if (valid && rank == 0) wc[wv][dg] = (uint32_t)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[dg] + (uint32_t)rank;
+      for (int w = 0; w < wv; ++w) pos += wc[w][dg];
+      const uint32_t v = vin[e];
+      kout[pos] = k;
+      vout[pos] = v;
+      if (pout) {
+        for (int q = 0; q < pq; ++q) pout[(int64_t)pos * pq + q] = pin[(int64_t)v * pq + q];
+        cout[pos] = cin[v];
+      }
+    }
+    __syncthreads();
+    if (valid && rank == 0) {
+      atomicAdd(&run[dg], wc[wv][dg]);
+      wc[wv][dg] = 0;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 void BinSort::init(int64_t n, int key_bits) {
@@ -185,10 +285,60 @@ void BinSort::init(int64_t n, int key_bits) {
   vtmp_.allocate(n);
 }
 
+void BinSort::set_segments(const std::vector<int64_t>& starts, int nbuck) {
+  if (starts.size() < 2 || starts.front() != 0 || starts.back() != n_)
+    throw InvalidArgument("sort segments must cover the keys");
+  if (nbuck < 1 || nbuck > RS_MAXB) throw InvalidArgument("segment digits must fit 11 bits");
+  std::vector<int64_t> tb, te;
+  std::vector<int> t0;
+  for (size_t i = 0; i + 1 < starts.size(); ++i) {
+    if (starts[i + 1] < starts[i]) throw InvalidArgument("sort segments must be ordered");
+    t0.push_back((int)tb.size());
+    for (int64_t b = starts[i]; b < starts[i + 1]; b += RS_TILE) {
+      tb.push_back(b);
+      te.push_back(std::min<int64_t>(starts[i + 1], b + RS_TILE));
+    }
+  }
+  t0.push_back((int)tb.size());
+  nsegs_ = (int)starts.size() - 1;
+  nseg_tiles_ = (int)tb.size();
+  seg_buck_ = nbuck;
+  tile_beg_.allocate(std::max<size_t>(1, tb.size()));
+  tile_end_.allocate(std::max<size_t>(1, te.size()));
+  if (!tb.empty()) {
+    tile_beg_.upload(tb.data(), tb.size());
+    tile_end_.upload(te.data(), te.size());
+  }
+  seg_tile0_.allocate(t0.size());
+  seg_tile0_.upload(t0.data(), t0.size());
+  seg_start_.allocate(starts.size());
+  seg_start_.upload(starts.data(), starts.size());
+  seg_cnt_.allocate(std::max<size_t>(1, (size_t)nseg_tiles_ * nbuck));
+  seg_ = true;
+}
+
 void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out,
                    hipStream_t s, const SortPayload* payload) {
   if (payload && (payload->rec_words <= 0 || payload->rec_words % 4 != 0))
     throw InvalidArgument("sort payload records must be whole 16-byte words");
+  if (seg_) {
+    int dbits = 1;
+    while ((1 << dbits) < seg_buck_) ++dbits;
+    if (nseg_tiles_ > 0) {
+      hipLaunchKernelGGL(k_rs_seg_upsweep, dim3(nseg_tiles_), dim3(RS_T), 0, s, keys, tile_beg_.get(),
+                         tile_end_.get(), (uint32_t)seg_buck_, seg_cnt_.get());
+      hipLaunchKernelGGL(k_rs_seg_scan, dim3(nsegs_), dim3(RS_T), 0, s, seg_cnt_.get(), seg_tile0_.get(),
+                         seg_start_.get(), seg_buck_);
+      hipLaunchKernelGGL(k_rs_seg_scatter, dim3(nseg_tiles_), dim3(RS_T), 0, s, keys, vals, tile_beg_.get(),
+                         tile_end_.get(), dbits, (uint32_t)seg_buck_, seg_cnt_.get(), keys_out, vals_out,
+                         payload ? static_cast<const uint4*>(payload->rec_in) : nullptr,
+                         payload ? static_cast<uint4*>(payload->rec_out) : nullptr,
+                         payload ? payload->rec_words / 4 : 0, payload ? payload->cell_in : nullptr,
+                         payload ? payload->cell_out : nullptr);
+      FRESCO_HIP_CHECK(hipGetLastError());
+    }
+    return;
+  }
   const uint32_t* kin = keys;
   const uint32_t* vin = vals;
   const int nbuck = 1 << dbits_;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 
@@ -29,6 +30,11 @@ class BinSort {
   // with a payload, its records are placed in the output order too
   void sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s,
             const SortPayload* payload = nullptr);
+  // Keys seg * nbuck + digit whose input is already grouped by seg in
+  // segments [starts[i], starts[i+1]) (starts: host, nseg + 1 entries):
+  // one stable pass on the digit inside every segment gives the same output
+  // as sort().  Call once after init; sort() then takes this path.
+  void set_segments(const std::vector<int64_t>& starts, int nbuck);
 
  private:
   int64_t n_ = 0;
@@ -37,6 +43,12 @@ class BinSort {
   int nblk_ = 1;
   int nseg_ = 1;
   DeviceBuffer<uint32_t> cnt_, tops_, ktmp_, vtmp_;
+  // segmented mode
+  bool seg_ = false;
+  int nseg_tiles_ = 0, nsegs_ = 0, seg_buck_ = 0;
+  DeviceBuffer<int64_t> tile_beg_, tile_end_, seg_start_;
+  DeviceBuffer<int> seg_tile0_;
+  DeviceBuffer<uint32_t> seg_cnt_;
 };
 
 }  // namespace fresco_hip
