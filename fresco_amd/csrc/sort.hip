@@ -214,55 +214,123 @@ __global__ void __launch_bounds__(RS_T)
   }
 }
 
-// k_rs_scatter for a segment tile: running offsets from the tile-major table
+// k_rs_scatter for a segment tile, staged: every 1024-key chunk is first
+// ordered by digit in LDS (stable: the wave-multisplit ranks), then written
+// out run by run, so consecutive threads write consecutive places of one
+// digit's run (coalesced) instead of one scattered element each.  Dynamic
+// LDS: 7 nbuck words of counts + the staged keys, values and payload.
+constexpr int SEG_CHUNK = 1024;
+
 __global__ void __launch_bounds__(RS_T)
     k_rs_seg_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                      const int64_t* __restrict__ tb, const int64_t* __restrict__ te, int dbits, uint32_t nbuck,
                      const uint32_t* __restrict__ offs, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                      const uint4* __restrict__ pin, uint4* __restrict__ pout, int pq, const uint32_t* __restrict__ cin,
                      uint32_t* __restrict__ cout) {
-  __shared__ uint32_t run[RS_MAXB];
-  __shared__ uint32_t wc[RS_T / 64][RS_MAXB];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nb = (int)nbuck;
+  uint4* sp = reinterpret_cast<uint4*>(smem_raw);                   // [SEG_CHUNK * pq]
+  uint32_t* sk = reinterpret_cast<uint32_t*>(sp + SEG_CHUNK * pq);  // [SEG_CHUNK]
+  uint32_t* sv = sk + SEG_CHUNK;                                    // [SEG_CHUNK]
+  uint32_t* sc = sv + SEG_CHUNK;                                    // [SEG_CHUNK]
+  uint32_t* run = sc + SEG_CHUNK;                                   // [nb] global offsets
+  uint32_t* cc = run + nb;                                          // [nb] chunk counts
+  uint32_t* lofs = cc + nb;                                         // [nb] chunk digit starts
+  uint32_t* wc = lofs + nb;                                         // [4][nb]
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
-  for (int d = t; d < (int)nbuck; d += RS_T) {
-    run[d] = offs[(int64_t)blockIdx.x * nbuck + d];
+  for (int d = t; d < nb; d += RS_T) {
+    run[d] = offs[(int64_t)blockIdx.x * nb + d];
+    cc[d] = 0;
 #pragma unroll
-    for (int w = 0; w < RS_T / 64; ++w) wc[w][d] = 0;
+    for (int w = 0; w < RS_T / 64; ++w) wc[w * nb + d] = 0;
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int64_t beg = tb[blockIdx.x], end = te[blockIdx.x];
-  for (int64_t r0 = beg; r0 < end; r0 += RS_T) {
-    const int64_t e = r0 + t;
-    const bool valid = e < end;
-    const uint32_t k = valid ? kin[e] : 0u;
-    const uint32_t dg = k % nbuck;
-    uint64_t same = __ballot(valid);
-    for (int b = 0; b < dbits; ++b) {
-      const bool bit = (dg >> b) & 1u;
-      const uint64_t m = __ballot(bit);
-      same &= bit ? m : ~m;
+  for (int64_t c0 = beg; c0 < end; c0 += SEG_CHUNK) {
+    const int clen = (int)(end - c0 < SEG_CHUNK ? end - c0 : SEG_CHUNK);
+    // ranks within the chunk: RS_ROUNDS-style rounds of 256 keys
+    uint32_t dg[SEG_CHUNK / RS_T], lr[SEG_CHUNK / RS_T];
+#pragma unroll
+    for (int r = 0; r < SEG_CHUNK / RS_T; ++r) {
+      const int i = r * RS_T + t;
+      const bool valid = i < clen;
+      const uint32_t k = valid ? kin[c0 + i] : 0u;
+      const uint32_t d = k % nbuck;
+      uint64_t same = __ballot(valid);
+      for (int b = 0; b < dbits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        same &= bit ? m : ~m;
+      }
+      const int rank = __popcll(same & lt);
+      if (valid && rank == 0) wc[wv * nb + d] = (uint32_t)__popcll(same);
+      __syncthreads();
+      uint32_t pos = 0;
+      if (valid) {
+        pos = cc[d] + (uint32_t)rank;
+        for (int w = 0; w < wv; ++w) pos += wc[w * nb + d];
+      }
+      __syncthreads();
+      if (valid && rank == 0) {
+        atomicAdd(&cc[d], wc[wv * nb + d]);
+        wc[wv * nb + d] = 0;
+      }
+      dg[r] = valid ? d : 0xffffffffu;
+      lr[r] = pos;
+      __syncthreads();
     }
-    const int rank = __popcll(same & lt);
-    if (valid && rank == 0) wc[wv][dg] = (uint32_t)__popcll(same);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[dg] + (uint32_t)rank;
-      for (int w = 0; w < wv; ++w) pos += wc[w][dg];
-      const uint32_t v = vin[e];
-      kout[pos] = k;
-      vout[pos] = v;
-      if (pout) {
-        for (int q = 0; q < pq; ++q) pout[(int64_t)pos * pq + q] = pin[(int64_t)v * pq + q];
-        cout[pos] = cin[v];
+    // digit starts of the chunk (exclusive scan of cc, one thread per
+    // digit range)
+    {
+      const int per = (nb + RS_T - 1) / RS_T;
+      const int lo = t * per < nb ? t * per : nb;
+      const int hi = lo + per < nb ? lo + per : nb;
+      uint32_t ssum = 0;
+      for (int d = lo; d < hi; ++d) ssum += cc[d];
+      __shared__ uint32_t shs[RS_T];
+      uint32_t total;
+      uint32_t r0 = block_exclusive_scan(ssum, shs, total);
+      for (int d = lo; d < hi; ++d) {
+        lofs[d] = r0;
+        r0 += cc[d];
       }
     }
     __syncthreads();
-    if (valid && rank == 0) {
-      atomicAdd(&run[dg], wc[wv][dg]);
-      wc[wv][dg] = 0;
+    // stage in digit order
+#pragma unroll
+    for (int r = 0; r < SEG_CHUNK / RS_T; ++r) {
+      const int i = r * RS_T + t;
+      if (dg[r] != 0xffffffffu) {
+        const uint32_t slot = lofs[dg[r]] + lr[r];
+        const uint32_t v = vin[c0 + i];
+        sk[slot] = kin[c0 + i];
+        sv[slot] = v;
+        if (pout) {
+          for (int q = 0; q < pq; ++q) sp[slot * pq + q] = pin[(int64_t)v * pq + q];
+          sc[slot] = cin[v];
+        }
+      }
+    }
+    __syncthreads();
+    // write runs: slot -> run[digit] + (slot - lofs[digit])
+    for (int i = t; i < clen; i += RS_T) {
+      const uint32_t k = sk[i];
+      const uint32_t d = k % nbuck;
+      const uint32_t pos = run[d] + ((uint32_t)i - lofs[d]);
+      kout[pos] = k;
+      vout[pos] = sv[i];
+      if (pout) {
+        for (int q = 0; q < pq; ++q) pout[(int64_t)pos * pq + q] = sp[i * pq + q];
+        cout[pos] = sc[i];
+      }
+    }
+    __syncthreads();
+    for (int d = t; d < nb; d += RS_T) {
+      run[d] += cc[d];
+      cc[d] = 0;
     }
     __syncthreads();
   }
@@ -321,7 +389,9 @@ void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_ou
                    hipStream_t s, const SortPayload* payload) {
   if (payload && (payload->rec_words <= 0 || payload->rec_words % 4 != 0))
     throw InvalidArgument("sort payload records must be whole 16-byte words");
-  if (seg_) {
+  const size_t seg_lds = (size_t)SEG_CHUNK * (16 * (payload ? payload->rec_words / 4 : 0) + 12) +
+                        7 * (size_t)seg_buck_ * 4;
+  if (seg_ && seg_lds <= 64 * 1024) {
     int dbits = 1;
     while ((1 << dbits) < seg_buck_) ++dbits;
     if (nseg_tiles_ > 0) {
@@ -329,7 +399,7 @@ void BinSort::sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_ou
                          tile_end_.get(), (uint32_t)seg_buck_, seg_cnt_.get());
       hipLaunchKernelGGL(k_rs_seg_scan, dim3(nsegs_), dim3(RS_T), 0, s, seg_cnt_.get(), seg_tile0_.get(),
                          seg_start_.get(), seg_buck_);
-      hipLaunchKernelGGL(k_rs_seg_scatter, dim3(nseg_tiles_), dim3(RS_T), 0, s, keys, vals, tile_beg_.get(),
+      hipLaunchKernelGGL(k_rs_seg_scatter, dim3(nseg_tiles_), dim3(RS_T), seg_lds, s, keys, vals, tile_beg_.get(),
                          tile_end_.get(), dbits, (uint32_t)seg_buck_, seg_cnt_.get(), keys_out, vals_out,
                          payload ? static_cast<const uint4*>(payload->rec_in) : nullptr,
                          payload ? static_cast<uint4*>(payload->rec_out) : nullptr,

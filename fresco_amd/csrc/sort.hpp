@@ -33,7 +33,8 @@ class BinSort {
   // Keys seg * nbuck + digit whose input is already grouped by seg in
   // segments [starts[i], starts[i+1]) (starts: host, nseg + 1 entries):
   // one stable pass on the digit inside every segment gives the same output
-  // as sort().  Call once after init; sort() then takes this path.
+  // as sort().  Call once after init; sort() then takes this path when its
+  // staging fits 64 KB of LDS (else the passes over the combined keys).
   void set_segments(const std::vector<int64_t>& starts, int nbuck);
 
  private:
