@@ -233,25 +233,29 @@ __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, 
   double f = gg - i0;
   double sf = ceil(f - 0.5 * m.w);
   off = f - sf;  // in [w/2 - 1, w/2]
-  long long ic = (long long)(isfinite(i0) ? i0 : 0.0) + (long long)(isfinite(sf) ? sf : 0.0);
-  ic %= m.nf[a];
-  if (ic < 0) ic += m.nf[a];
-  cell = (int)ic;
+  // i0 in [0, nf] and sf in [-w/2, 1 - w/2]: one wrap step suffices; the
+  // remainder only for grids narrower than the kernel
+  int ic = (int)(isfinite(i0) ? i0 : 0.0) + (int)(isfinite(sf) ? sf : 0.0);
+  if (ic >= m.nf[a])
+    ic -= m.nf[a];
+  else if (ic < 0)
+    ic += m.nf[a];
+  if (ic < 0 || ic >= m.nf[a]) {
+    ic %= m.nf[a];
+    if (ic < 0) ic += m.nf[a];
+  }
+  cell = ic;
 }
 
-// bin key of every point (pattern * nbins + tile of its first support cell)
-// and, in the original order, its record (fine-grid offsets and weight) and
-// packed first cell, which the sort carries to the sorted order
+// bin key of point j (b0 + tile of its first support cell), its record
+// (fine-grid offsets and weight) and its packed first cell
 template <typename T>
-__global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m, uint32_t* __restrict__ key,
-                              uint32_t* __restrict__ idx, const uint32_t* __restrict__ pat, uint32_t nbins,
-                              const T* __restrict__ c, uint32_t* __restrict__ cell_out,
-                              typename Vec4<T>::type* __restrict__ rec_out) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  uint32_t b = pat ? pat[j] * nbins : 0u, mul = 1;
+__device__ __forceinline__ uint32_t point_record(const double* __restrict__ x, int64_t j, const MapDev& m,
+                                                 uint32_t b0, const T* __restrict__ c, uint32_t& packed,
+                                                 typename Vec4<T>::type& r) {
+  uint32_t b = b0, mul = 1;
   double offs[3] = {0, 0, 0};
-  uint32_t packed = 0;
+  packed = 0;
   for (int a = 0; a < m.d; ++a) {
     int cell;
     map_component(x[j * m.d + a], m, a, cell, offs[a]);
@@ -261,15 +265,204 @@ __global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m,
     b += mul * (uint32_t)t;
     mul *= (uint32_t)m.ntile[a];
   }
-  key[j] = b;
-  idx[j] = (uint32_t)j;
-  typename Vec4<T>::type r;
   r.x = (T)offs[0];
   r.y = (T)offs[1];
   r.z = (T)offs[2];
   r.w = c ? c[j] : T(1);
+  return b;
+}
+
+// bin key of every point (pattern * nbins + tile of its first support cell)
+// and, in the original order, its record and packed first cell, which the
+// sort carries to the sorted order
+template <typename T>
+__global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m, uint32_t* __restrict__ key,
+                              uint32_t* __restrict__ idx, const uint32_t* __restrict__ pat, uint32_t nbins,
+                              const T* __restrict__ c, uint32_t* __restrict__ cell_out,
+                              typename Vec4<T>::type* __restrict__ rec_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t packed;
+  typename Vec4<T>::type r;
+  key[j] = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, packed, r);
+  idx[j] = (uint32_t)j;
   cell_out[j] = packed;
   rec_out[j] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Direct binning: one counting pass when all bins of the plan (batch * nbins)
+// fit PB_MAXB and the plan is not segmented.
+//   k_bin_tiles   per tile of PB_TILE points: key, record and packed cell in
+//                 the original order, and the tile's bin counts (tile-major,
+//                 cnt[tile * nb + bin], so every later access is coalesced)
+//   k_bin_groups  per group of gsz tiles and bin: exclusive prefix of the
+//                 counts over the group's tiles, in place; the group total
+//                 to gsum[group * nb + bin]
+//   k_bin_bases   one workgroup: exclusive prefix of the group totals over
+//                 the groups (in place) and of the bin totals over the bins
+//                 (the bin starts)
+//   k_bin_place   per tile: running offsets start + group prefix + tile
+//                 prefix, then rounds of 256 points in index order, each
+//                 wavefront ranking its lanes among equal bins with one
+//                 ballot per key bit; records move to their sorted places
+// The result is the stable sort by key (index order inside every bin) that
+// the radix sort gives, without its extra key and index arrays.
+// ---------------------------------------------------------------------------
+constexpr int PB_T = 256;
+constexpr int PB_ROUNDS = 8;
+constexpr int PB_TILE = PB_T * PB_ROUNDS;
+constexpr int PB_MAXB = 2048;
+constexpr int PB_GMAX = 64;  // at most this many tile groups
+
+template <typename T>
+__global__ void __launch_bounds__(PB_T)
+    k_bin_tiles(const double* __restrict__ x, int64_t n, MapDev m, const uint32_t* __restrict__ pat,
+                uint32_t nbins, const T* __restrict__ c, int nb, uint32_t* __restrict__ key,
+                uint32_t* __restrict__ cell_out, typename Vec4<T>::type* __restrict__ rec_out,
+                uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[PB_MAXB];
+  const int t = threadIdx.x;
+  for (int b = t; b < nb; b += PB_T) h[b] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * PB_TILE;
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int64_t j = base + r * PB_T + t;
+    if (j < n) {
+      uint32_t packed;
+      typename Vec4<T>::type rc;
+      const uint32_t k = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, packed, rc);
+      key[j] = k;
+      cell_out[j] = packed;
+      rec_out[j] = rc;
+      atomicAdd(&h[k], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = t; b < nb; b += PB_T) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
+}
+
+// grid (ceil(nb / PB_T), groups); gsz a multiple of 16
+__global__ void __launch_bounds__(PB_T)
+    k_bin_groups(uint32_t* __restrict__ cnt, int ntiles, int nb, int gsz, uint32_t* __restrict__ gsum) {
+  const int b = blockIdx.x * PB_T + threadIdx.x;
+  if (b >= nb) return;
+  const int t0 = blockIdx.y * gsz;
+  const int t1 = t0 + gsz < ntiles ? t0 + gsz : ntiles;
+  uint32_t s = 0;
+  for (int c0 = t0; c0 < t1; c0 += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = c0 + i < t1 ? cnt[(int64_t)(c0 + i) * nb + b] : 0u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (c0 + i < t1) cnt[(int64_t)(c0 + i) * nb + b] = s;
+      s += v[i];
+    }
+  }
+  gsum[(int64_t)blockIdx.y * nb + b] = s;
+}
+
+constexpr int PB_BT = PB_MAXB / 2;  // threads of k_bin_bases: two bins each
+
+__global__ void __launch_bounds__(PB_BT)
+    k_bin_bases(uint32_t* __restrict__ gsum, int ngroups, int nb, int64_t n, int* __restrict__ start) {
+  __shared__ uint32_t tot[PB_MAXB];
+  __shared__ uint32_t sh[PB_BT];
+  const int t = threadIdx.x;
+  for (int b = t; b < PB_MAXB; b += PB_BT) {
+    uint32_t acc = 0;
+    if (b < nb) {
+      int g = 0;
+      for (; g + 8 <= ngroups; g += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = gsum[(int64_t)(g + i) * nb + b];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          gsum[(int64_t)(g + i) * nb + b] = acc;
+          acc += v[i];
+        }
+      }
+      for (; g < ngroups; ++g) {
+        const uint32_t v = gsum[(int64_t)g * nb + b];
+        gsum[(int64_t)g * nb + b] = acc;
+        acc += v;
+      }
+    }
+    tot[b] = acc;
+  }
+  __syncthreads();
+  const uint32_t a0 = tot[2 * t], a1 = tot[2 * t + 1];
+  sh[t] = a0 + a1;
+  __syncthreads();
+  for (int off = 1; off < PB_BT; off <<= 1) {
+    const uint32_t a = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  const uint32_t ex = sh[t] - (a0 + a1);
+  if (2 * t < nb) start[2 * t] = (int)ex;
+  if (2 * t + 1 < nb) start[2 * t + 1] = (int)(ex + a0);
+  if (t == 0) start[nb] = (int)n;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PB_T)
+    k_bin_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ cell_in,
+                const typename Vec4<T>::type* __restrict__ rec_in, int64_t n, int nb, int dbits,
+                const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ gsum, int gsz,
+                const int* __restrict__ start, uint32_t* __restrict__ idx, uint32_t* __restrict__ cell_out,
+                typename Vec4<T>::type* __restrict__ rec_out) {
+  __shared__ uint32_t run[PB_MAXB];
+  __shared__ uint32_t wc[PB_T / 64][PB_MAXB];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  const int tile = blockIdx.x;
+  const int64_t grow = (int64_t)(tile / gsz) * nb, trow = (int64_t)tile * nb;
+  for (int b = t; b < nb; b += PB_T) {
+    run[b] = (uint32_t)start[b] + gsum[grow + b] + cnt[trow + b];
+#pragma unroll
+    for (int w = 0; w < PB_T / 64; ++w) wc[w][b] = 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int64_t base = (int64_t)tile * PB_TILE;
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int64_t j = base + r * PB_T + t;
+    const bool valid = j < n;
+    uint32_t k = 0, cl = 0;
+    typename Vec4<T>::type rc;
+    if (valid) {
+      k = key[j];
+      cl = cell_in[j];
+      rc = rec_in[j];
+    }
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+      const bool bit = (k >> b) & 1u;
+      const uint64_t mk = __ballot(bit);
+      same &= bit ? mk : ~mk;
+    }
+    const int rank = __popcll(same & lt);
+    if (valid && rank == 0) wc[wv][k] = (uint32_t)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[k] + (uint32_t)rank;
+      for (int w = 0; w < wv; ++w) pos += wc[w][k];
+      idx[pos] = (uint32_t)j;
+      cell_out[pos] = cl;
+      rec_out[pos] = rc;
+    }
+    __syncthreads();
+    if (valid && rank == 0) {
+      atomicAdd(&run[k], wc[wv][k]);
+      wc[wv][k] = 0;
+    }
+    __syncthreads();
+  }
 }
 
 template <typename K>
@@ -312,7 +505,6 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   es = e;
   n = npts;
   const int64_t tb = (int64_t)g.nbins * g.batch;
-  idx_in.allocate(n);
   idx.allocate(n);
   bin_start.allocate(tb + 1);
   scell.allocate(n);
@@ -320,8 +512,30 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   scell_in.allocate(n);
   rec_in.allocate(n);
   key.allocate(n);
+  direct = tb <= PB_MAXB;
+  if (direct) {
+    ntiles_ = (int)((n + PB_TILE - 1) / PB_TILE);
+    gsz_ = 16 * (int)((ntiles_ + 16 * PB_GMAX - 1) / (16 * PB_GMAX));
+    ngroups_ = (ntiles_ + gsz_ - 1) / gsz_;
+    tcnt_.allocate((size_t)ntiles_ * tb);
+    gsum_.allocate((size_t)ngroups_ * tb);
+  } else {
+    init_sorter();
+  }
+}
+
+template <typename T>
+void SortedPoints<T>::init_sorter() {
+  direct = false;
+  idx_in.allocate(n);
   key_sorted.allocate(n);
-  sorter.init(n, g.key_bits);
+  sorter.init(n, geom.key_bits);
+}
+
+template <typename T>
+void SortedPoints<T>::set_segments(const std::vector<int64_t>& starts) {
+  if (direct) init_sorter();
+  sorter.set_segments(starts, geom.nbins);
 }
 
 template <typename T>
@@ -330,6 +544,19 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
   const unsigned thr = 256;
   if ((geom.batch > 1) != (pat != nullptr)) throw InvalidArgument("pattern indices needed exactly for batched plans");
   const int tb = geom.nbins * geom.batch;
+  if (direct) {
+    int dbits = 1;
+    while ((1 << dbits) < tb) ++dbits;
+    hipLaunchKernelGGL(k_bin_tiles<T>, dim3(ntiles_), dim3(PB_T), 0, s, x, n, m, pat, (uint32_t)geom.nbins, c, tb,
+                       key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
+    hipLaunchKernelGGL(k_bin_groups, dim3((tb + PB_T - 1) / PB_T, ngroups_), dim3(PB_T), 0, s, tcnt_.get(), ntiles_,
+                       tb, gsz_, gsum_.get());
+    hipLaunchKernelGGL(k_bin_bases, dim3(1), dim3(PB_BT), 0, s, gsum_.get(), ngroups_, tb, n, bin_start.get());
+    hipLaunchKernelGGL(k_bin_place<T>, dim3(ntiles_), dim3(PB_T), 0, s, key.get(), scell_in.get(), rec_in.get(), n,
+                       tb, dbits, tcnt_.get(), gsum_.get(), gsz_, bin_start.get(), idx.get(), scell.get(), rec.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return;
+  }
   // records computed once, in the original (coalesced) order, and carried to
   // the sorted order by the sort's last pass
   hipLaunchKernelGGL(k_bin_records<T>, dim3(blocks_for(n, thr)), dim3(thr), 0, s, x, n, m, key.get(), idx_in.get(),
@@ -808,6 +1035,8 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
 
 template <typename T>
 __global__ void k_sum_parts(const T* __restrict__ part, int parts, int64_t n, T* __restrict__ grid);
+template <typename T>
+static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s);
 
 template <typename T, int D, int WMAX>
 static void launch_spread(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {

@@ -80,7 +80,7 @@ class SortedPoints {
   // batched plans whose points are stored pattern by pattern: pattern p owns
   // points [starts[p], starts[p+1]); the bin sort then runs one pass per
   // pattern instead of sorting the combined keys
-  void set_segments(const std::vector<int64_t>& starts) { sorter.set_segments(starts, geom.nbins); }
+  void set_segments(const std::vector<int64_t>& starts);
   // x: device float64 [n*d]; c: device T[n] weights (may be null -> 1);
   // pat: device pattern index of every point (geom.batch > 1), else null.
   void set(const double* x, const CoordMap& m, const T* c, hipStream_t s, const uint32_t* pat = nullptr);
@@ -89,12 +89,20 @@ class SortedPoints {
   GridGeom geom;
   EsParams es;
   DeviceBuffer<uint32_t> key, key_sorted, idx_in, idx;  // idx: sorted -> original
+  // direct: one counting pass over tiles (all bins of the plan fit one
+  // 11-bit digit, no segments); else the radix sort over the keys
+  bool direct = false;
   BinSort sorter;                                       // stable LSD radix sort (sort.hip)
   DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
   DeviceBuffer<uint32_t> scell_in;                      // the same in the original order
   DeviceBuffer<R4> rec_in;
+
+ private:
+  void init_sorter();
+  int ntiles_ = 0, gsz_ = 16, ngroups_ = 0;
+  DeviceBuffer<uint32_t> tcnt_, gsum_;  // direct binning: per-tile counts, per-group totals
 };
 
 template <typename T>
