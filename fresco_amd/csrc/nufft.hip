@@ -313,33 +313,41 @@ constexpr int PB_T = 256;
 constexpr int PB_ROUNDS = 8;
 constexpr int PB_TILE = PB_T * PB_ROUNDS;
 constexpr int PB_MAXB = 2048;
-constexpr int PB_GMAX = 64;  // at most this many tile groups
+constexpr int PB_GMAX = 32;  // at most this many tile groups
+
+constexpr int PB_CT = 1024;  // threads of k_bin_tiles: PB_TILE / PB_CT points each
 
 template <typename T>
-__global__ void __launch_bounds__(PB_T)
+__global__ void __launch_bounds__(PB_CT)
     k_bin_tiles(const double* __restrict__ x, int64_t n, MapDev m, const uint32_t* __restrict__ pat,
                 uint32_t nbins, const T* __restrict__ c, int nb, uint32_t* __restrict__ key,
                 uint32_t* __restrict__ cell_out, typename Vec4<T>::type* __restrict__ rec_out,
                 uint32_t* __restrict__ cnt) {
+  constexpr int PER = PB_TILE / PB_CT;
   __shared__ uint32_t h[PB_MAXB];
   const int t = threadIdx.x;
-  for (int b = t; b < nb; b += PB_T) h[b] = 0;
-  __syncthreads();
+  for (int b = t; b < nb; b += PB_CT) h[b] = 0;
   const int64_t base = (int64_t)blockIdx.x * PB_TILE;
-  for (int r = 0; r < PB_ROUNDS; ++r) {
-    const int64_t j = base + r * PB_T + t;
+  uint32_t k[PER], packed[PER];
+  typename Vec4<T>::type rc[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int64_t j = base + r * PB_CT + t;
+    if (j < n) k[r] = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, packed[r], rc[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int64_t j = base + r * PB_CT + t;
     if (j < n) {
-      uint32_t packed;
-      typename Vec4<T>::type rc;
-      const uint32_t k = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, packed, rc);
-      key[j] = k;
-      cell_out[j] = packed;
-      rec_out[j] = rc;
-      atomicAdd(&h[k], 1u);
+      key[j] = k[r];
+      cell_out[j] = packed[r];
+      rec_out[j] = rc[r];
+      atomicAdd(&h[k[r]], 1u);
     }
   }
   __syncthreads();
-  for (int b = t; b < nb; b += PB_T) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
+  for (int b = t; b < nb; b += PB_CT) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
 }
 
 // grid (ceil(nb / PB_T), groups); gsz a multiple of 16
@@ -368,41 +376,39 @@ constexpr int PB_BT = PB_MAXB / 2;  // threads of k_bin_bases: two bins each
 __global__ void __launch_bounds__(PB_BT)
     k_bin_bases(uint32_t* __restrict__ gsum, int ngroups, int nb, int64_t n, int* __restrict__ start) {
   __shared__ uint32_t tot[PB_MAXB];
-  __shared__ uint32_t sh[PB_BT];
+  __shared__ uint32_t wsum[PB_BT / 64];
   const int t = threadIdx.x;
   for (int b = t; b < PB_MAXB; b += PB_BT) {
     uint32_t acc = 0;
     if (b < nb) {
-      int g = 0;
-      for (; g + 8 <= ngroups; g += 8) {
-        uint32_t v[8];
+      for (int g0 = 0; g0 < ngroups; g0 += 16) {
+        uint32_t v[16];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = gsum[(int64_t)(g + i) * nb + b];
+        for (int i = 0; i < 16; ++i) v[i] = g0 + i < ngroups ? gsum[(g0 + i) * nb + b] : 0u;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          gsum[(int64_t)(g + i) * nb + b] = acc;
+        for (int i = 0; i < 16; ++i) {
+          if (g0 + i < ngroups) gsum[(g0 + i) * nb + b] = acc;
           acc += v[i];
         }
-      }
-      for (; g < ngroups; ++g) {
-        const uint32_t v = gsum[(int64_t)g * nb + b];
-        gsum[(int64_t)g * nb + b] = acc;
-        acc += v;
       }
     }
     tot[b] = acc;
   }
   __syncthreads();
+  // exclusive scan of the bin totals, two consecutive bins per thread:
+  // wavefront scan with lane shifts, then the wavefront totals
   const uint32_t a0 = tot[2 * t], a1 = tot[2 * t + 1];
-  sh[t] = a0 + a1;
-  __syncthreads();
-  for (int off = 1; off < PB_BT; off <<= 1) {
-    const uint32_t a = t >= off ? sh[t - off] : 0u;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
+  const int lane = t & 63;
+  uint32_t inc = a0 + a1;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += u;
   }
-  const uint32_t ex = sh[t] - (a0 + a1);
+  if (lane == 63) wsum[t >> 6] = inc;
+  __syncthreads();
+  uint32_t ex = inc - (a0 + a1);
+  for (int w = 0; w < (t >> 6); ++w) ex += wsum[w];
   if (2 * t < nb) start[2 * t] = (int)ex;
   if (2 * t + 1 < nb) start[2 * t + 1] = (int)(ex + a0);
   if (t == 0) start[nb] = (int)n;
@@ -420,7 +426,11 @@ __global__ void __launch_bounds__(PB_T)
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
-  const int tile = blockIdx.x;
+  // workgroups go round-robin to the 8 XCDs; each XCD takes a contiguous
+  // range of tiles, so the runs a bin receives from neighbouring tiles (which
+  // share cache lines of the outputs) are written through one L2
+  const int nt = gridDim.x, xq = nt >> 3, xr = nt & 7, xc = blockIdx.x & 7;
+  const int tile = xc * xq + (xc < xr ? xc : xr) + (blockIdx.x >> 3);
   const int64_t grow = (int64_t)(tile / gsz) * nb, trow = (int64_t)tile * nb;
   for (int b = t; b < nb; b += PB_T) {
     run[b] = (uint32_t)start[b] + gsum[grow + b] + cnt[trow + b];
@@ -430,16 +440,27 @@ __global__ void __launch_bounds__(PB_T)
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int64_t base = (int64_t)tile * PB_TILE;
+  // every round's inputs are loaded up front: the rounds then wait on LDS
+  // and barriers only
+  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS];
+  typename Vec4<T>::type rcs[PB_ROUNDS];
+#pragma unroll
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int64_t j = base + r * PB_T + t;
+    ks[r] = 0;
+    cls[r] = 0;
+    if (j < n) {
+      ks[r] = key[j];
+      cls[r] = cell_in[j];
+      rcs[r] = rec_in[j];
+    }
+  }
+#pragma unroll
   for (int r = 0; r < PB_ROUNDS; ++r) {
     const int64_t j = base + r * PB_T + t;
     const bool valid = j < n;
-    uint32_t k = 0, cl = 0;
-    typename Vec4<T>::type rc;
-    if (valid) {
-      k = key[j];
-      cl = cell_in[j];
-      rc = rec_in[j];
-    }
+    const uint32_t k = ks[r], cl = cls[r];
+    const typename Vec4<T>::type rc = rcs[r];
     uint64_t same = __ballot(valid);
     for (int b = 0; b < dbits; ++b) {
       const bool bit = (k >> b) & 1u;
@@ -547,7 +568,7 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
   if (direct) {
     int dbits = 1;
     while ((1 << dbits) < tb) ++dbits;
-    hipLaunchKernelGGL(k_bin_tiles<T>, dim3(ntiles_), dim3(PB_T), 0, s, x, n, m, pat, (uint32_t)geom.nbins, c, tb,
+    hipLaunchKernelGGL(k_bin_tiles<T>, dim3(ntiles_), dim3(PB_CT), 0, s, x, n, m, pat, (uint32_t)geom.nbins, c, tb,
                        key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
     hipLaunchKernelGGL(k_bin_groups, dim3((tb + PB_T - 1) / PB_T, ngroups_), dim3(PB_T), 0, s, tcnt_.get(), ntiles_,
                        tb, gsz_, gsum_.get());
