@@ -992,12 +992,15 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
-    // FRESCO_SPREAD3=reg | halo | halo:<slices> pins the choice (reproducible
+    // FRESCO_SPREAD3=reg | halo | halo:<slices> | halo4:<slices> pins the choice (reproducible
     // rounding across processes) instead of timing the candidates
     if (const char* env = std::getenv("FRESCO_SPREAD3")) {
       const std::string v(env);
       if (v == "reg") {
         mode = REG;
+      } else if (v.rfind("halo4", 0) == 0) {
+        mode = HALO4;
+        if (v.size() > 6) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 6)));
       } else if (v.rfind("halo", 0) == 0) {
         mode = HALO;
         if (v.size() > 5) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 5)));
@@ -1019,6 +1022,8 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
   if (geom.tile[0] == 16 && geom.tile[1] == 16 && geom.tile[2] == 8) cands.push_back({REG, 1});
   for (int sl : {1, 2, 3, 4, 6, 8, 12, 16})
     if (sl <= max_halo_parts) cands.push_back({HALO, sl});
+  for (int sl : {1, 2, 3, 4})
+    if (sl <= max_halo_parts) cands.push_back({HALO4, sl});
   hipEvent_t e0, e1;
   FRESCO_HIP_CHECK(hipEventCreate(&e0));
   FRESCO_HIP_CHECK(hipEventCreate(&e1));
@@ -1690,6 +1695,108 @@ __global__ void __launch_bounds__(64)
   for (int e = lane; e < HT_ELEMS; e += 64) out[e] = tile[e];
 }
 
+// The same spreading with four wavefronts per halo tile: wavefront q owns the
+// tile's z-planes congruent to q mod 4, so of every point's 8 support planes
+// it adds exactly two (offsets j1 = (q - z0) mod 4 and j1 + 4) and no two
+// wavefronts ever touch one cell (plain read-modify-write stays race-free).
+// Per chunk of H4_CHUNK points the workgroup evaluates the kernel values once
+// into LDS and forms the 64 (x, y) products c * wx * wy of every point; a
+// wavefront then reads its lane's product and multiplies by its two z values.
+// Four times the wavefronts per LDS tile of the one-wavefront kernel; the
+// products, their order and the per-cell point order are the same, so the
+// halo tiles are bitwise equal.
+constexpr int H4_CHUNK = 32;  // 256 threads = H4_CHUNK points x 8 kernel indices
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(256)
+    k_spread3_halo4(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                    const int* __restrict__ bin_start, SpreadDev<T> a, T* __restrict__ halo) {
+  constexpr int HS_Y = HaloShape<TY>::HS_Y;
+  constexpr int HT_ELEMS = HaloShape<TY>::ELEMS;
+  constexpr int PLANE = HS_X * HS_Y;
+  __shared__ __align__(16) T tile[HT_ELEMS];
+  __shared__ T wk[H4_CHUNK][3 * HW];  // kernel values: x (0-7), y (8-15), z (16-23)
+  __shared__ T wxy[H4_CHUNK][64];     // c * wx[ja] * wy[jb] at lane ja + 8 jb
+  __shared__ uint32_t sch[H4_CHUNK];  // packed first cells
+  __shared__ T cw[H4_CHUNK];          // point weights
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int q = tid >> 6;
+  const int w = a.w;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.y) / gridDim.y);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.y + 1)) / gridDim.y);
+  T* out = halo + ((int64_t)b * gridDim.y + blockIdx.y) * HT_ELEMS;
+  if (beg == end) {
+    for (int e = tid; e < HT_ELEMS; e += 256) out[e] = T(0);
+    return;
+  }
+  for (int e = tid; e < HT_ELEMS; e += 256) tile[e] = T(0);
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  const int ja = lane & 7, jb = lane >> 3;
+  const int loff = jb * HS_X + ja;  // this lane's (x, y) cell of a support
+  // thread (point tid / 8, index tid % 8) evaluates the x, y and z kernel
+  // values; its point's record (and, for tid < H4_CHUNK, the packed cell) is
+  // loaded one chunk ahead, so the loads overlap the previous chunk's adds
+  const int kp = tid >> 3, kj = tid & 7;
+  typename Vec4<T>::type rn;
+  uint32_t scn = 0;
+  rn.x = rn.y = rn.z = rn.w = T(0);
+  if (beg + kp < end) rn = rec[beg + kp];
+  if (tid < H4_CHUNK && beg + tid < end) scn = scell[beg + tid];
+  for (int c0 = beg; c0 < end; c0 += H4_CHUNK) {
+    const int cnt = min(H4_CHUNK, end - c0);
+    const typename Vec4<T>::type r = rn;
+    const uint32_t scc = scn;
+    if (c0 + H4_CHUNK + kp < end) rn = rec[c0 + H4_CHUNK + kp];
+    if (tid < H4_CHUNK && c0 + H4_CHUNK + tid < end) scn = scell[c0 + H4_CHUNK + tid];
+    if (kp < cnt) {
+      const bool on = kj < w;
+      wk[kp][kj] = on ? es_phi<T>((T(kj) - r.x) * a.k.two_over_w, a.k) : T(0);
+      wk[kp][HW + kj] = on ? es_phi<T>((T(kj) - r.y) * a.k.two_over_w, a.k) : T(0);
+      wk[kp][2 * HW + kj] = on ? es_phi<T>((T(kj) - r.z) * a.k.two_over_w, a.k) : T(0);
+      if (kj == 0) cw[kp] = r.w;
+    }
+    if (tid < cnt) sch[tid] = scc;
+    __syncthreads();
+    for (int e = tid; e < cnt * 64; e += 256) {
+      const int pp = e >> 6, l = e & 63;
+      wxy[pp][l] = wk[pp][l & 7] * wk[pp][8 + (l >> 3)] * cw[pp];
+    }
+    // lane pp of every wavefront: its point's tile offset of plane j1 and
+    // the two z values this wavefront applies
+    int base = 0;
+    T z0 = T(0), z1 = T(0);
+    if (lane < cnt) {
+      const uint32_t sc = sch[lane];
+      const int ux = (int)(sc & cmask) - t0[0];
+      const int uy = (int)((sc >> a.cbits) & cmask) - t0[1];
+      const int uz = (int)((sc >> (2 * a.cbits)) & cmask) - t0[2];
+      const int j1 = (q - uz) & 3;
+      base = ((uz + j1) * HS_Y + uy) * HS_X + ux;
+      z0 = wk[lane][2 * HW + j1];
+      z1 = wk[lane][2 * HW + j1 + 4];
+    }
+    __syncthreads();
+    for (int pp = 0; pp < cnt; ++pp) {
+      const int bp = __builtin_amdgcn_readlane(base, pp);
+      const T za = lane_read(z0, pp), zb = lane_read(z1, pp);
+      const T v = wxy[pp][lane];
+      T* c = tile + bp + loff;
+      c[0] += v * za;
+      c[4 * PLANE] += v * zb;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < HT_ELEMS; e += 256) out[e] = tile[e];
+}
+
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
 // covering fine-grid coordinate c along dimension a, -1 terminated
 template <typename T, int TY>
@@ -1864,7 +1971,7 @@ static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
 
 template <typename T>
 static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
-  if (!sp.halo_ok || sp.mode != Spreader<T>::HALO) return false;
+  if (!sp.halo_ok || (sp.mode != Spreader<T>::HALO && sp.mode != Spreader<T>::HALO4)) return false;
   SpreadDev<T> a;
   const GridGeom& g = sp.geom;
   for (int q = 0; q < 3; ++q) {
@@ -1883,12 +1990,21 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   const bool t8 = g.tile[1] == 8;
   {
     ProfScope ps(ST_SPREAD, s);
-    if (t8)
-      hipLaunchKernelGGL((k_spread3_halo<T, 8>), dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(),
-                         p.rec.get(), p.bin_start.get(), a, sp.halo_buf.get());
-    else
-      hipLaunchKernelGGL((k_spread3_halo<T, 16>), dim3(g.nbins, sp.halo_parts), dim3(64), 0, s, p.scell.get(),
-                         p.rec.get(), p.bin_start.get(), a, sp.halo_buf.get());
+    const dim3 grd(g.nbins, sp.halo_parts);
+    if (sp.mode == Spreader<T>::HALO4) {
+      if (t8)
+        hipLaunchKernelGGL((k_spread3_halo4<T, 8>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(),
+                           p.bin_start.get(), a, sp.halo_buf.get());
+      else
+        hipLaunchKernelGGL((k_spread3_halo4<T, 16>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(),
+                           p.bin_start.get(), a, sp.halo_buf.get());
+    } else if (t8) {
+      hipLaunchKernelGGL((k_spread3_halo<T, 8>), grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
+                         a, sp.halo_buf.get());
+    } else {
+      hipLaunchKernelGGL((k_spread3_halo<T, 16>), grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
+                         a, sp.halo_buf.get());
+    }
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);
@@ -2034,7 +2150,11 @@ __device__ __forceinline__ T sum8_last(T v) {
   return v;
 }
 
-template <typename T, bool DERIV>
+// SRX, RY != 0: the region's row stride and rows per plane fixed at compile
+// time (the 16 x 8 x 8 and 16 x 16 x 8 bins), so the 64 cell reads of a
+// point become immediate offsets of one address (paired LDS reads, no
+// per-read address arithmetic); 0: taken from a
+template <typename T, bool DERIV, int SRX = 0, int RY = 0>
 __global__ void __launch_bounds__(G8_THREADS)
     k_interp3_g8(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
                  const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
@@ -2062,23 +2182,25 @@ __global__ void __launch_bounds__(G8_THREADS)
     int gx = t0[0] + lx;
     if (gx >= a.nf[0]) gx -= a.nf[0];
     if (gx >= a.nf[0]) gx -= a.nf[0];
-    const int nrows = a.ry * a.rz;
+    const int nrows = (RY ? RY : a.ry) * a.rz;
     for (int row = tid >> 5; row < nrows; row += G8_THREADS / 32) {
-      const int ly = row % a.ry;
-      const int lz = row / a.ry;
+      const int ly = row % (RY ? RY : a.ry);
+      const int lz = row / (RY ? RY : a.ry);
       int gy = t0[1] + ly;
       if (gy >= a.nf[1]) gy -= a.nf[1];
       if (gy >= a.nf[1]) gy -= a.nf[1];
       int gz = t0[2] + lz;
       if (gz >= a.nf[2]) gz -= a.nf[2];
       if (gz >= a.nf[2]) gz -= a.nf[2];
-      if (lx < a.rx) reg[(lz * a.ry + ly) * a.srx + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+      if (lx < a.rx)
+        reg[(lz * (RY ? RY : a.ry) + ly) * (SRX ? SRX : a.srx) + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
     }
   }
   __syncthreads();
   const int ja = tid & 7;
   const int grp = tid >> 3;
-  const int zs = a.ry * a.srx;
+  const int srx = SRX ? SRX : a.srx;
+  const int zs = (RY ? RY : a.ry) * srx;
   const int npts = end - beg;
   // every group runs the same number of iterations (DPP needs all lanes)
   const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
@@ -2129,7 +2251,7 @@ __global__ void __launch_bounds__(G8_THREADS)
       group8_bcast<T>(dwya, dwy);
       group8_bcast<T>(dwza, dwz);
     }
-    const T* col = reg + (l0[2] * a.ry + l0[1]) * a.srx + l0[0] + ja;
+    const T* col = reg + (l0[2] * (RY ? RY : a.ry) + l0[1]) * srx + l0[0] + ja;
     // all 8 x 8 cells are read (weights beyond w are zero and the region
     // is staged 8 - 1 cells wider than the tile), so the loads can be batched
     T sA = T(0), sB = T(0), sC = T(0);
@@ -2138,7 +2260,7 @@ __global__ void __launch_bounds__(G8_THREADS)
       T pA = T(0), pB = T(0);
 #pragma unroll
       for (int jb = 0; jb < 8; ++jb) {
-        const T g = col[jc * zs + jb * a.srx];
+        const T g = col[jc * zs + jb * srx];
         pA += g * wy[jb];
         if (DERIV) pB += g * dwy[jb];
       }
@@ -2196,8 +2318,16 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS");
   // enough workgroups to fill the GPU: split bins when there are few
   const int chunks = std::max(1, std::min(8, (2048 + g.nbins - 1) / g.nbins));
-  hipLaunchKernelGGL((k_interp3_g8<T, DERIV>), dim3(g.nbins, chunks), dim3(G8_THREADS), lds, s, p.scell.get(),
-                     p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+  const dim3 grd(g.nbins, chunks);
+  if (a.srx == 23 && a.ry == 15 && a.rx == 23)
+    hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 23, 15>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
+                       p.idx.get(), p.bin_start.get(), grid, a, out);
+  else if (a.srx == 23 && a.ry == 23 && a.rx == 23)
+    hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 23, 23>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
+                       p.idx.get(), p.bin_start.get(), grid, a, out);
+  else
+    hipLaunchKernelGGL((k_interp3_g8<T, DERIV>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
+                       p.idx.get(), p.bin_start.get(), grid, a, out);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 

@@ -133,7 +133,9 @@ class Spreader {
   DeviceBuffer<T> part_buf_c;  // [cparts][ncells][2]
   DeviceBuffer<T> part_buf;  // [parts][ncells] / [lparts][batch][ncells] partial grids
   // input-driven halo spreader (3D, w <= 8)
-  enum Mode { REG = 0, HALO = 1 };
+  // HALO4: the halo spreader with four wavefronts per tile (z-planes split
+  // by residue mod 4)
+  enum Mode { REG = 0, HALO = 1, HALO4 = 2 };
   mutable int mode = REG;
   bool halo_ok = false;
   mutable bool tuned = true;   // false: time the candidates on the first spread

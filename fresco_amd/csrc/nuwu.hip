@@ -316,7 +316,7 @@ class NUwU final : public Potential {
 
   void spread_config(int* mode, int* parts) const override {
     *mode = spreader_.mode;
-    *parts = spreader_.mode == Spreader<T>::HALO ? spreader_.halo_parts : spreader_.parts;
+    *parts = spreader_.mode == Spreader<T>::REG ? spreader_.parts : spreader_.halo_parts;
   }
 
   void target_grads(double* dS0, double* dV, hipStream_t s) override {

@@ -191,7 +191,7 @@ class BasePotential:
         parts = ctypes.c_int()
         _lib.check(lib.fresco_pot_spread_config(self._h, ctypes.byref(mode), ctypes.byref(parts)))
         return {"nf": list(nf), "w": int(w.value), "beta": float(beta.value), "tile": list(tile),
-                "spreader": {0: "tiles", 1: "halo"}.get(mode.value), "spread_parts": int(parts.value)}
+                "spreader": {0: "tiles", 1: "halo", 2: "halo4"}.get(mode.value), "spread_parts": int(parts.value)}
 
     def __del__(self):
         h = getattr(self, "_h", None)

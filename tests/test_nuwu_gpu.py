@@ -234,6 +234,30 @@ def test_halo_spreader_parity_and_determinism(NUwU, prec, eps, tol):
     assert E == E2 and np.array_equal(g, g2)
 
 
+@pytest.mark.parametrize("prec,eps,tol", [("fp32", 1e-6, 1e-5), ("fp64", 1e-7, 1e-6)])
+def test_halo_spreader_wavefront_layouts_agree(NUwU, prec, eps, tol, monkeypatch):
+    """The halo spreader with one wavefront per tile and with four (z-planes
+    split by residue mod 4), at one and three point slices per bin: parity
+    with the exact sums, and the two layouts bitwise equal at equal slices
+    (same products, same per-cell order)."""
+    radii, Sk, V, p = make_case(3, 21, 5000, seed=43)
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
+    E0, g0 = ref.get_energy_gradient(p)
+    res = {}
+    for mode in ("halo:1", "halo:3", "halo4:1", "halo4:3"):
+        monkeypatch.setenv("FRESCO_SPREAD3", mode)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), eps, precision=prec)
+        info = pot.plan_info()
+        assert info["spreader"] == mode.split(":")[0] and info["spread_parts"] == int(mode[-1])
+        E, g = pot.get_energy_gradient(p)
+        assert abs(E - E0) / abs(E0) < tol
+        assert rel(g, g0) < tol
+        res[mode] = (E, g)
+    for sl in ("1", "3"):
+        assert res["halo:" + sl][0] == res["halo4:" + sl][0]
+        assert np.array_equal(res["halo:" + sl][1], res["halo4:" + sl][1])
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64-adjoint"])
 def test_adjoint_gradient_with_weighted_k0(NUwU, prec):
     """V(k = 0) != 0 on a small 1D grid: the k = 0 term carries rho_0 = N but
