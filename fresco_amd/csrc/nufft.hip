@@ -992,7 +992,7 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
-    // FRESCO_SPREAD3=reg | halo | halo:<slices> | halo4:<slices> pins the choice (reproducible
+    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] pins the choice (reproducible
     // rounding across processes) instead of timing the candidates
     if (const char* env = std::getenv("FRESCO_SPREAD3")) {
       const std::string v(env);
@@ -1695,30 +1695,35 @@ __global__ void __launch_bounds__(64)
   for (int e = lane; e < HT_ELEMS; e += 64) out[e] = tile[e];
 }
 
-// The same spreading with four wavefronts per halo tile: wavefront q owns the
-// tile's z-planes congruent to q mod 4, so of every point's 8 support planes
-// it adds exactly two (offsets j1 = (q - z0) mod 4 and j1 + 4) and no two
+// The same spreading with NW wavefronts per halo tile (launched with 4 and
+// 32-point chunks: measured best of NW 4 / 8 and CH 32 / 64 on the 3D
+// N = 2^20 configuration): wavefront q owns the tile's
+// z-planes congruent to q mod NW, so of every point's 8 support planes it
+// adds exactly 8 / NW (offsets j1 = (q - z0) mod NW, j1 + NW, ...) and no two
 // wavefronts ever touch one cell (plain read-modify-write stays race-free).
-// Per chunk of H4_CHUNK points the workgroup evaluates the kernel values once
-// into LDS and forms the 64 (x, y) products c * wx * wy of every point; a
-// wavefront then reads its lane's product and multiplies by its two z values.
-// Four times the wavefronts per LDS tile of the one-wavefront kernel; the
-// products, their order and the per-cell point order are the same, so the
-// halo tiles are bitwise equal.
-constexpr int H4_CHUNK = 32;  // 256 threads = H4_CHUNK points x 8 kernel indices
-
-template <typename T, int TY>
-__global__ void __launch_bounds__(256)
-    k_spread3_halo4(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
-                    const int* __restrict__ bin_start, SpreadDev<T> a, T* __restrict__ halo) {
+// Per chunk of CH points the workgroup evaluates the kernel values once into
+// LDS (thread = point x kernel index, records loaded a chunk ahead) and
+// forms the 64 (x, y) products c * wx * wy of every point;
+// a wavefront then reads its lane's product and multiplies by its z values.
+// More wavefronts per LDS tile than the one-wavefront kernel; the products,
+// their order and the per-cell point order are the same, so the halo tiles
+// are bitwise equal.
+template <typename T, int TY, int NW, int CH>
+__global__ void __launch_bounds__(64 * NW)
+    k_spread3_halo_mw(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                      const int* __restrict__ bin_start, SpreadDev<T> a, T* __restrict__ halo) {
+  constexpr int NT = 64 * NW;
+  constexpr int KR = CH * 8 / NT;  // (point, kernel index) pairs per thread
+  static_assert(KR * NT == CH * 8, "chunk must be a multiple of NT / 8 points");
+  constexpr int NP = 8 / NW;   // planes per point and wavefront
   constexpr int HS_Y = HaloShape<TY>::HS_Y;
   constexpr int HT_ELEMS = HaloShape<TY>::ELEMS;
   constexpr int PLANE = HS_X * HS_Y;
   __shared__ __align__(16) T tile[HT_ELEMS];
-  __shared__ T wk[H4_CHUNK][3 * HW];  // kernel values: x (0-7), y (8-15), z (16-23)
-  __shared__ T wxy[H4_CHUNK][64];     // c * wx[ja] * wy[jb] at lane ja + 8 jb
-  __shared__ uint32_t sch[H4_CHUNK];  // packed first cells
-  __shared__ T cw[H4_CHUNK];          // point weights
+  __shared__ T wk[CH][3 * HW];  // kernel values: x (0-7), y (8-15), z (16-23)
+  __shared__ T wxy[CH][64];     // c * wx[ja] * wy[jb] at lane ja + 8 jb
+  __shared__ uint32_t sch[CH];  // packed first cells
+  __shared__ T cw[CH];          // point weights
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int q = tid >> 6;
@@ -1731,70 +1736,88 @@ __global__ void __launch_bounds__(256)
   const int end = beg0 + (int)(((int64_t)len * (blockIdx.y + 1)) / gridDim.y);
   T* out = halo + ((int64_t)b * gridDim.y + blockIdx.y) * HT_ELEMS;
   if (beg == end) {
-    for (int e = tid; e < HT_ELEMS; e += 256) out[e] = T(0);
+    for (int e = tid; e < HT_ELEMS; e += NT) out[e] = T(0);
     return;
   }
-  for (int e = tid; e < HT_ELEMS; e += 256) tile[e] = T(0);
+  for (int e = tid; e < HT_ELEMS; e += NT) tile[e] = T(0);
   int t0[3];
   t0[0] = (b % a.ntile[0]) * a.tile[0];
   t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
   t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
   const int ja = lane & 7, jb = lane >> 3;
   const int loff = jb * HS_X + ja;  // this lane's (x, y) cell of a support
-  // thread (point tid / 8, index tid % 8) evaluates the x, y and z kernel
-  // values; its point's record (and, for tid < H4_CHUNK, the packed cell) is
-  // loaded one chunk ahead, so the loads overlap the previous chunk's adds
-  const int kp = tid >> 3, kj = tid & 7;
-  typename Vec4<T>::type rn;
+  const int kj = tid & 7;
+  typename Vec4<T>::type rn[KR];
   uint32_t scn = 0;
-  rn.x = rn.y = rn.z = rn.w = T(0);
-  if (beg + kp < end) rn = rec[beg + kp];
-  if (tid < H4_CHUNK && beg + tid < end) scn = scell[beg + tid];
-  for (int c0 = beg; c0 < end; c0 += H4_CHUNK) {
-    const int cnt = min(H4_CHUNK, end - c0);
-    const typename Vec4<T>::type r = rn;
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int kp = k * (NT / 8) + (tid >> 3);
+    rn[k].x = rn[k].y = rn[k].z = rn[k].w = T(0);
+    if (beg + kp < end) rn[k] = rec[beg + kp];
+  }
+  if (tid < CH && beg + tid < end) scn = scell[beg + tid];
+  for (int c0 = beg; c0 < end; c0 += CH) {
+    const int cnt = min(CH, end - c0);
     const uint32_t scc = scn;
-    if (c0 + H4_CHUNK + kp < end) rn = rec[c0 + H4_CHUNK + kp];
-    if (tid < H4_CHUNK && c0 + H4_CHUNK + tid < end) scn = scell[c0 + H4_CHUNK + tid];
-    if (kp < cnt) {
-      const bool on = kj < w;
-      wk[kp][kj] = on ? es_phi<T>((T(kj) - r.x) * a.k.two_over_w, a.k) : T(0);
-      wk[kp][HW + kj] = on ? es_phi<T>((T(kj) - r.y) * a.k.two_over_w, a.k) : T(0);
-      wk[kp][2 * HW + kj] = on ? es_phi<T>((T(kj) - r.z) * a.k.two_over_w, a.k) : T(0);
-      if (kj == 0) cw[kp] = r.w;
+    if (tid < CH && c0 + CH + tid < end) scn = scell[c0 + CH + tid];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int kp = k * (NT / 8) + (tid >> 3);
+      const typename Vec4<T>::type r = rn[k];
+      if (c0 + CH + kp < end) rn[k] = rec[c0 + CH + kp];
+      if (kp < cnt) {
+        const bool on = kj < w;
+        wk[kp][kj] = on ? es_phi<T>((T(kj) - r.x) * a.k.two_over_w, a.k) : T(0);
+        wk[kp][HW + kj] = on ? es_phi<T>((T(kj) - r.y) * a.k.two_over_w, a.k) : T(0);
+        wk[kp][2 * HW + kj] = on ? es_phi<T>((T(kj) - r.z) * a.k.two_over_w, a.k) : T(0);
+        if (kj == 0) cw[kp] = r.w;
+      }
     }
     if (tid < cnt) sch[tid] = scc;
     __syncthreads();
-    for (int e = tid; e < cnt * 64; e += 256) {
+    for (int e = tid; e < cnt * 64; e += NT) {
       const int pp = e >> 6, l = e & 63;
       wxy[pp][l] = wk[pp][l & 7] * wk[pp][8 + (l >> 3)] * cw[pp];
     }
-    // lane pp of every wavefront: its point's tile offset of plane j1 and
-    // the two z values this wavefront applies
-    int base = 0;
-    T z0 = T(0), z1 = T(0);
-    if (lane < cnt) {
-      const uint32_t sc = sch[lane];
-      const int ux = (int)(sc & cmask) - t0[0];
-      const int uy = (int)((sc >> a.cbits) & cmask) - t0[1];
-      const int uz = (int)((sc >> (2 * a.cbits)) & cmask) - t0[2];
-      const int j1 = (q - uz) & 3;
-      base = ((uz + j1) * HS_Y + uy) * HS_X + ux;
-      z0 = wk[lane][2 * HW + j1];
-      z1 = wk[lane][2 * HW + j1 + 4];
+    // lanes of every wavefront, one point each: the tile offset of the
+    // point's plane j1 and the z values this wavefront applies
+    int base[CH / 64 + 1];
+    T zv[CH / 64 + 1][NP];
+#pragma unroll
+    for (int h = 0; h * 64 < CH; ++h) {
+      const int pp = h * 64 + lane;
+      base[h] = 0;
+#pragma unroll
+      for (int m = 0; m < NP; ++m) zv[h][m] = T(0);
+      if (pp < cnt) {
+        const uint32_t sc = sch[pp];
+        const int ux = (int)(sc & cmask) - t0[0];
+        const int uy = (int)((sc >> a.cbits) & cmask) - t0[1];
+        const int uz = (int)((sc >> (2 * a.cbits)) & cmask) - t0[2];
+        const int j1 = (q - uz) & (NW - 1);
+        base[h] = ((uz + j1) * HS_Y + uy) * HS_X + ux;
+#pragma unroll
+        for (int m = 0; m < NP; ++m) zv[h][m] = wk[pp][2 * HW + j1 + m * NW];
+      }
     }
     __syncthreads();
-    for (int pp = 0; pp < cnt; ++pp) {
-      const int bp = __builtin_amdgcn_readlane(base, pp);
-      const T za = lane_read(z0, pp), zb = lane_read(z1, pp);
-      const T v = wxy[pp][lane];
-      T* c = tile + bp + loff;
-      c[0] += v * za;
-      c[4 * PLANE] += v * zb;
+#pragma unroll
+    for (int h = 0; h * 64 < CH; ++h) {
+      const int lim = min(cnt - h * 64, 64);
+      for (int i = 0; i < lim; ++i) {
+        const int bp = __builtin_amdgcn_readlane(base[h], i);
+        T za[NP];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) za[m] = lane_read(zv[h][m], i);
+        const T v = wxy[h * 64 + i][lane];
+        T* c = tile + bp + loff;
+#pragma unroll
+        for (int m = 0; m < NP; ++m) c[m * NW * PLANE] += v * za[m];
+      }
     }
     __syncthreads();
   }
-  for (int e = tid; e < HT_ELEMS; e += 256) out[e] = tile[e];
+  for (int e = tid; e < HT_ELEMS; e += NT) out[e] = tile[e];
 }
 
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
@@ -1959,6 +1982,7 @@ static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
     hipLaunchKernelGGL(k_spread2_halo<T>, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64),
                        sizeof(T) * (size_t)helems, s, p.scell.get(), p.rec.get(), p.bin_start.get(), a, hs, hrows,
                        sp.halo_buf.get());
+#undef FRESCO_HALO_MW
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);
@@ -1971,7 +1995,7 @@ static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
 
 template <typename T>
 static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p, T* grid, hipStream_t s) {
-  if (!sp.halo_ok || (sp.mode != Spreader<T>::HALO && sp.mode != Spreader<T>::HALO4)) return false;
+  if (!sp.halo_ok || sp.mode == Spreader<T>::REG) return false;
   SpreadDev<T> a;
   const GridGeom& g = sp.geom;
   for (int q = 0; q < 3; ++q) {
@@ -1991,13 +2015,14 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   {
     ProfScope ps(ST_SPREAD, s);
     const dim3 grd(g.nbins, sp.halo_parts);
+#define FRESCO_HALO_MW(TY_, CH_)                                                                          \
+  hipLaunchKernelGGL((k_spread3_halo_mw<T, TY_, 4, CH_>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(), \
+                     p.bin_start.get(), a, sp.halo_buf.get())
     if (sp.mode == Spreader<T>::HALO4) {
       if (t8)
-        hipLaunchKernelGGL((k_spread3_halo4<T, 8>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(),
-                           p.bin_start.get(), a, sp.halo_buf.get());
+        FRESCO_HALO_MW(8, 32);
       else
-        hipLaunchKernelGGL((k_spread3_halo4<T, 16>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(),
-                           p.bin_start.get(), a, sp.halo_buf.get());
+        FRESCO_HALO_MW(16, 32);
     } else if (t8) {
       hipLaunchKernelGGL((k_spread3_halo<T, 8>), grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
                          a, sp.halo_buf.get());
@@ -2005,6 +2030,7 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
       hipLaunchKernelGGL((k_spread3_halo<T, 16>), grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
                          a, sp.halo_buf.get());
     }
+#undef FRESCO_HALO_MW
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);

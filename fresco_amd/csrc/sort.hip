@@ -33,6 +33,15 @@ constexpr int RS_TILE = RS_T * RS_ROUNDS;  // keys per tile
 constexpr int RS_MAXB = 2048;              // buckets of an 11-bit digit
 constexpr int RS_SEG = 4096;               // scan segment
 
+// Workgroups go round-robin to the 8 XCDs; this maps workgroup b of nt to a
+// tile so that every XCD takes a contiguous range of tiles: neighbouring
+// tiles write neighbouring places of a run (shared cache lines), which then
+// fill in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nt) {
+  const int xq = nt >> 3, xr = nt & 7, xc = b & 7;
+  return xc * xq + (xc < xr ? xc : xr) + (b >> 3);
+}
+
 __global__ void __launch_bounds__(RS_T)
     k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t dmask, int nbuck,
                  uint32_t* __restrict__ cnt, int nblk) {
@@ -120,8 +129,9 @@ __global__ void __launch_bounds__(RS_T)
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   for (int d = t; d < nbuck; d += RS_T) {
-    const int64_t i = (int64_t)d * nblk + blockIdx.x;
+    const int64_t i = (int64_t)d * nblk + tile;
     run[d] = offs[i] + tops[i / RS_SEG];
   }
   for (int d = t; d < nbuck; d += RS_T) {
@@ -130,7 +140,7 @@ __global__ void __launch_bounds__(RS_T)
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t base = (int64_t)tile * RS_TILE;
   for (int r = 0; r < RS_ROUNDS; ++r) {
     const int64_t e = base + (int64_t)r * RS_T + t;
     const bool valid = e < n;
@@ -237,18 +247,19 @@ __global__ void __launch_bounds__(RS_T)
   uint32_t* cc = run + nb;                                          // [nb] chunk counts
   uint32_t* lofs = cc + nb;                                         // [nb] chunk digit starts
   uint32_t* wc = lofs + nb;                                         // [4][nb]
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
   for (int d = t; d < nb; d += RS_T) {
-    run[d] = offs[(int64_t)blockIdx.x * nb + d];
+    run[d] = offs[(int64_t)tile * nb + d];
     cc[d] = 0;
 #pragma unroll
     for (int w = 0; w < RS_T / 64; ++w) wc[w * nb + d] = 0;
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
-  const int64_t beg = tb[blockIdx.x], end = te[blockIdx.x];
+  const int64_t beg = tb[tile], end = te[tile];
   for (int64_t c0 = beg; c0 < end; c0 += SEG_CHUNK) {
     const int clen = (int)(end - c0 < SEG_CHUNK ? end - c0 : SEG_CHUNK);
     // ranks within the chunk: RS_ROUNDS-style rounds of 256 keys
