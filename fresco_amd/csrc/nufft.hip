@@ -650,6 +650,42 @@ __device__ __forceinline__ T wave_sum63(T v) {
   return v;
 }
 
+// out[j] = value of lane (lane & ~7) + j, j < 8 (ds_swizzle bit mode:
+// and_mask 0x18, or_mask j)
+template <int J>
+__device__ __forceinline__ int swz8(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, 0x18 | (J << 5));
+}
+template <typename T>
+__device__ __forceinline__ void group8_bcast(T v, T* out);
+template <>
+__device__ __forceinline__ void group8_bcast<float>(float v, float* out) {
+  const int b = __float_as_int(v);
+  out[0] = __int_as_float(swz8<0>(b));
+  out[1] = __int_as_float(swz8<1>(b));
+  out[2] = __int_as_float(swz8<2>(b));
+  out[3] = __int_as_float(swz8<3>(b));
+  out[4] = __int_as_float(swz8<4>(b));
+  out[5] = __int_as_float(swz8<5>(b));
+  out[6] = __int_as_float(swz8<6>(b));
+  out[7] = __int_as_float(swz8<7>(b));
+}
+template <>
+__device__ __forceinline__ void group8_bcast<double>(double v, double* out) {
+  const long long bb = __double_as_longlong(v);
+  const int lo = (int)(bb & 0xffffffffLL), hi = (int)(bb >> 32);
+  int l[8], h[8];
+  l[0] = swz8<0>(lo); h[0] = swz8<0>(hi);
+  l[1] = swz8<1>(lo); h[1] = swz8<1>(hi);
+  l[2] = swz8<2>(lo); h[2] = swz8<2>(hi);
+  l[3] = swz8<3>(lo); h[3] = swz8<3>(hi);
+  l[4] = swz8<4>(lo); h[4] = swz8<4>(hi);
+  l[5] = swz8<5>(lo); h[5] = swz8<5>(hi);
+  l[6] = swz8<6>(lo); h[6] = swz8<6>(hi);
+  l[7] = swz8<7>(lo); h[7] = swz8<7>(hi);
+  for (int j = 0; j < 8; ++j) out[j] = __longlong_as_double(((long long)h[j] << 32) | (unsigned int)l[j]);
+}
+
 // ---------------------------------------------------------------------------
 // spreading: one wavefront owns one tile (LDS), owner-computes
 // ---------------------------------------------------------------------------
@@ -1720,10 +1756,9 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr int HT_ELEMS = HaloShape<TY>::ELEMS;
   constexpr int PLANE = HS_X * HS_Y;
   __shared__ __align__(16) T tile[HT_ELEMS];
-  __shared__ T wk[CH][3 * HW];  // kernel values: x (0-7), y (8-15), z (16-23)
-  __shared__ T wxy[CH][64];     // c * wx[ja] * wy[jb] at lane ja + 8 jb
-  __shared__ uint32_t sch[CH];  // packed first cells
-  __shared__ T cw[CH];          // point weights
+  __shared__ T wzs[CH][HW];                  // z kernel values
+  __shared__ __align__(16) T wxy[CH][64];     // c * wx[ja] * wy[jb] at lane ja + 8 jb
+  __shared__ uint32_t sch[CH];               // packed first cells
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int q = tid >> 6;
@@ -1760,25 +1795,29 @@ __global__ void __launch_bounds__(64 * NW)
     const int cnt = min(CH, end - c0);
     const uint32_t scc = scn;
     if (tid < CH && c0 + CH + tid < end) scn = scell[c0 + CH + tid];
+    // thread (point kp, index kj): its x, y, z kernel values; the 8 threads
+    // of a point share the x values by swizzle and thread kj writes row
+    // jb = kj of the point's products, (wx * wy) * c
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
       const int kp = k * (NT / 8) + (tid >> 3);
       const typename Vec4<T>::type r = rn[k];
       if (c0 + CH + kp < end) rn[k] = rec[c0 + CH + kp];
+      const bool on = kj < w && kp < cnt;
+      const T wxv = on ? es_phi<T>((T(kj) - r.x) * a.k.two_over_w, a.k) : T(0);
+      const T wyv = on ? es_phi<T>((T(kj) - r.y) * a.k.two_over_w, a.k) : T(0);
+      const T wzv = on ? es_phi<T>((T(kj) - r.z) * a.k.two_over_w, a.k) : T(0);
+      T wx8[8];
+      group8_bcast<T>(wxv, wx8);
       if (kp < cnt) {
-        const bool on = kj < w;
-        wk[kp][kj] = on ? es_phi<T>((T(kj) - r.x) * a.k.two_over_w, a.k) : T(0);
-        wk[kp][HW + kj] = on ? es_phi<T>((T(kj) - r.y) * a.k.two_over_w, a.k) : T(0);
-        wk[kp][2 * HW + kj] = on ? es_phi<T>((T(kj) - r.z) * a.k.two_over_w, a.k) : T(0);
-        if (kj == 0) cw[kp] = r.w;
+        T* row = &wxy[kp][8 * kj];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) row[j] = wx8[j] * wyv * r.w;
+        wzs[kp][kj] = wzv;
       }
     }
     if (tid < cnt) sch[tid] = scc;
     __syncthreads();
-    for (int e = tid; e < cnt * 64; e += NT) {
-      const int pp = e >> 6, l = e & 63;
-      wxy[pp][l] = wk[pp][l & 7] * wk[pp][8 + (l >> 3)] * cw[pp];
-    }
     // lanes of every wavefront, one point each: the tile offset of the
     // point's plane j1 and the z values this wavefront applies
     int base[CH / 64 + 1];
@@ -1797,10 +1836,9 @@ __global__ void __launch_bounds__(64 * NW)
         const int j1 = (q - uz) & (NW - 1);
         base[h] = ((uz + j1) * HS_Y + uy) * HS_X + ux;
 #pragma unroll
-        for (int m = 0; m < NP; ++m) zv[h][m] = wk[pp][2 * HW + j1 + m * NW];
+        for (int m = 0; m < NP; ++m) zv[h][m] = wzs[pp][j1 + m * NW];
       }
     }
-    __syncthreads();
 #pragma unroll
     for (int h = 0; h * 64 < CH; ++h) {
       const int lim = min(cnt - h * 64, 64);
@@ -2131,42 +2169,6 @@ static void launch_interp3_lane(const SortedPoints<T>& p, const T* grid, const d
 // ---------------------------------------------------------------------------
 
 constexpr int G8_THREADS = 256;
-
-// out[j] = value of lane (lane & ~7) + j, j < 8 (ds_swizzle bit mode:
-// and_mask 0x18, or_mask j)
-template <int J>
-__device__ __forceinline__ int swz8(int v) {
-  return __builtin_amdgcn_ds_swizzle(v, 0x18 | (J << 5));
-}
-template <typename T>
-__device__ __forceinline__ void group8_bcast(T v, T* out);
-template <>
-__device__ __forceinline__ void group8_bcast<float>(float v, float* out) {
-  const int b = __float_as_int(v);
-  out[0] = __int_as_float(swz8<0>(b));
-  out[1] = __int_as_float(swz8<1>(b));
-  out[2] = __int_as_float(swz8<2>(b));
-  out[3] = __int_as_float(swz8<3>(b));
-  out[4] = __int_as_float(swz8<4>(b));
-  out[5] = __int_as_float(swz8<5>(b));
-  out[6] = __int_as_float(swz8<6>(b));
-  out[7] = __int_as_float(swz8<7>(b));
-}
-template <>
-__device__ __forceinline__ void group8_bcast<double>(double v, double* out) {
-  const long long bb = __double_as_longlong(v);
-  const int lo = (int)(bb & 0xffffffffLL), hi = (int)(bb >> 32);
-  int l[8], h[8];
-  l[0] = swz8<0>(lo); h[0] = swz8<0>(hi);
-  l[1] = swz8<1>(lo); h[1] = swz8<1>(hi);
-  l[2] = swz8<2>(lo); h[2] = swz8<2>(hi);
-  l[3] = swz8<3>(lo); h[3] = swz8<3>(hi);
-  l[4] = swz8<4>(lo); h[4] = swz8<4>(hi);
-  l[5] = swz8<5>(lo); h[5] = swz8<5>(hi);
-  l[6] = swz8<6>(lo); h[6] = swz8<6>(hi);
-  l[7] = swz8<7>(lo); h[7] = swz8<7>(hi);
-  for (int j = 0; j < 8; ++j) out[j] = __longlong_as_double(((long long)h[j] << 32) | (unsigned int)l[j]);
-}
 
 template <typename T>
 __device__ __forceinline__ T sum8_last(T v) {
