@@ -1893,6 +1893,69 @@ __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__
   }
 }
 
+// k_merge_halo for P slices per bin: a cell covered by at most two tiles per
+// dimension (every cell unless a partial last tile sits next to the wrap)
+// issues all its up to 8 P halo loads at once and sums them in the same order
+// as k_merge_halo (z, y, x entries, then slices), so the grid is bitwise the
+// same; 32-bit offsets (the host checks the halo buffer size).
+template <typename T, int TY, int P>
+__global__ void __launch_bounds__(256)
+    k_merge_halo_p(const T* __restrict__ halo, const int* __restrict__ tabs, SpreadDev<T> a, T* __restrict__ grid) {
+  constexpr int HS_Y = HaloShape<TY>::HS_Y;
+  constexpr int HT = HaloShape<TY>::ELEMS;
+  const int nfx = a.nf[0], nfy = a.nf[1];
+  const int64_t n = (int64_t)nfx * nfy * a.nf[2];
+  const int4* tx = reinterpret_cast<const int4*>(tabs);
+  const int4* ty = tx + nfx;
+  const int4* tz = ty + nfy;
+  const int nt0 = a.ntile[0], nt01 = a.ntile[0] * a.ntile[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int gx = (int)(i % nfx);
+    const int64_t r = i / nfx;
+    const int gy = (int)(r % nfy);
+    const int gz = (int)(r / nfy);
+    const int4 ex = tx[gx], ey = ty[gy], ez = tz[gz];
+    T acc = T(0);
+    if (ex.z < 0 && ey.z < 0 && ez.z < 0) {
+      // per dimension: (bin part, cell part) of entries 0 and 1
+      const int bx[2] = {ex.x >> 6, ex.y >> 6}, ux[2] = {ex.x & 63, ex.y & 63};
+      const int by[2] = {(ey.x >> 6) * nt0, (ey.y >> 6) * nt0};
+      const int uy[2] = {(ey.x & 63) * HS_X, (ey.y & 63) * HS_X};
+      const int bz[2] = {(ez.x >> 6) * nt01, (ez.y >> 6) * nt01};
+      const int uz[2] = {(ez.x & 63) * (HS_Y * HS_X), (ez.y & 63) * (HS_Y * HS_X)};
+      const bool vx1 = ex.y >= 0, vy1 = ey.y >= 0, vz1 = ez.y >= 0;
+      T v[8][P];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int iz = c >> 2, iy = (c >> 1) & 1, ix = c & 1;
+        const bool ok = (ix == 0 || vx1) && (iy == 0 || vy1) && (iz == 0 || vz1);
+        const int base = (bx[ix] + by[iy] + bz[iz]) * (P * HT) + ux[ix] + uy[iy] + uz[iz];
+#pragma unroll
+        for (int sl = 0; sl < P; ++sl) v[c][sl] = ok ? halo[base + sl * HT] : T(0);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int iz = c >> 2, iy = (c >> 1) & 1, ix = c & 1;
+        if ((ix == 0 || vx1) && (iy == 0 || vy1) && (iz == 0 || vz1)) {
+#pragma unroll
+          for (int sl = 0; sl < P; ++sl) acc += v[c][sl];
+        }
+      }
+    } else {
+      const int exa[4] = {ex.x, ex.y, ex.z, ex.w}, eya[4] = {ey.x, ey.y, ey.z, ey.w};
+      const int eza[4] = {ez.x, ez.y, ez.z, ez.w};
+      for (int iz = 0; iz < 4 && eza[iz] >= 0; ++iz)
+        for (int iy = 0; iy < 4 && eya[iy] >= 0; ++iy)
+          for (int ix = 0; ix < 4 && exa[ix] >= 0; ++ix) {
+            const int bin = (exa[ix] >> 6) + nt0 * (eya[iy] >> 6) + nt01 * (eza[iz] >> 6);
+            const int off = ((eza[iz] & 63) * HS_Y + (eya[iy] & 63)) * HS_X + (exa[ix] & 63);
+            for (int sl = 0; sl < P; ++sl) acc += halo[(bin * P + sl) * HT + off];
+          }
+    }
+    grid[i] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2D input-driven spreading (w <= 8, real strengths): a bin's supports lie in
 // its (tile + 7)^2 halo tile; one wavefront per (bin, slice, pattern) adds its
@@ -2074,6 +2137,25 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   ProfScope pm(ST_MERGE, s);
   const size_t n = (size_t)g.ncells();
   const dim3 grd((unsigned)std::min<size_t>(blocks_for(n, 256), 16384));
+  const size_t hel = t8 ? HaloShape<8>::ELEMS : HaloShape<16>::ELEMS;
+  if (sp.halo_parts <= 4 && (size_t)g.nbins * sp.halo_parts * hel < (1u << 31)) {
+#define FRESCO_MERGE_P(TY_, P_)                                                                              \
+  hipLaunchKernelGGL((k_merge_halo_p<T, TY_, P_>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, \
+                     grid)
+    switch (sp.halo_parts * (t8 ? 1 : -1)) {
+      case 1: FRESCO_MERGE_P(8, 1); break;
+      case 2: FRESCO_MERGE_P(8, 2); break;
+      case 3: FRESCO_MERGE_P(8, 3); break;
+      case 4: FRESCO_MERGE_P(8, 4); break;
+      case -1: FRESCO_MERGE_P(16, 1); break;
+      case -2: FRESCO_MERGE_P(16, 2); break;
+      case -3: FRESCO_MERGE_P(16, 3); break;
+      default: FRESCO_MERGE_P(16, 4); break;
+    }
+#undef FRESCO_MERGE_P
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return true;
+  }
   if (t8)
     hipLaunchKernelGGL((k_merge_halo<T, 8>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(),
                        sp.halo_parts, a, grid);
