@@ -1,0 +1,57 @@
+// Pruned separable DFTs between a real 3D fine grid and the band of modes
+// |k_a| <= B that a structure-factor loss reads (the fine grid is sized from
+// that band, so B is small against the grid: 24 of 100 cells per dimension
+// for the 3D N = 2^20 configuration).  Each direction is three passes that
+// contract one axis against a twiddle matrix (batched small matrix products),
+// replacing a full real-to-complex / complex-to-real FFT of the grid plus the
+// zero fill of its spectrum:
+//   forward  grid[z][y][x] -> U1[z][y][kx] -> U2[z][ky][kx] -> S[kz][ky][kx]
+//   inverse  X[kz][ky][kx] -> Y1[z][ky][kx] -> Y2[z][y][kx] -> grid[z][y][x]
+// with kx in [0, B] (real input: S(-k) = conj S(k)) and ky, kz in [-B, B].
+// Signs and scaling are those of the FFTs they replace: forward e^{-2 pi i
+// k x / n}, inverse e^{+2 pi i k x / n} unnormalised, the inverse taking the
+// kx = 0 plane as it is (Hermitian by construction) and kx > 0 twice (their
+// conjugate partners -k).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nufft.hpp"
+
+namespace fresco_hip {
+
+template <typename T>
+class BandDft {
+ public:
+  // nf: fine grid (x fastest); band: B
+  void init(const int* nf, int band);
+  bool ready() const { return B_ >= 0; }
+  // largest band this path takes for a grid of n cells per dimension (the
+  // passes cost ~ n^3 B; beyond it the FFTs are cheaper)
+  static bool suits(const int* nf, int band) {
+    return band >= 0 && band <= 32 && nf[0] <= 128 && nf[1] <= 128 && nf[2] <= 128;
+  }
+  // grid (real, nf[2] x nf[1] x nf[0]) -> S (compact band spectrum); zero:
+  // optional compact array cleared by the same launches (every entry is
+  // written once)
+  void forward(const T* grid, Cplx<T>* S, hipStream_t s, Cplx<T>* zero = nullptr);
+  // X (compact band spectrum, kx = 0 plane Hermitian) -> grid
+  void inverse(const Cplx<T>* X, T* grid, hipStream_t s);
+  // compact layout: entries and the index of (kx >= 0, ky, kz)
+  int64_t size() const { return (int64_t)nb_ * (2 * B_ + 1) * (2 * B_ + 1); }
+  int64_t index(int kx, int ky, int kz) const {
+    return ((int64_t)(kz + B_) * (2 * B_ + 1) + (ky + B_)) * nb_ + kx;
+  }
+  int band() const { return B_; }
+
+ private:
+  int nf_[3] = {0, 0, 0};
+  int B_ = -1, nb_ = 0;
+  // twiddles of the six passes, [K][J] with J the contracted length
+  DeviceBuffer<Cplx<T>> wa_, wb_, wc_, wci_, wbi_, wai_;
+  DeviceBuffer<Cplx<T>> u1_, u2_;  // intermediates (the inverse reuses them)
+};
+
+}  // namespace fresco_hip

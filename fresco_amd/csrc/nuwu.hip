@@ -18,9 +18,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <string>
 
+#include "banddft.hpp"
 #include "nufft.hpp"
 #include "potential.hpp"
 
@@ -120,6 +122,7 @@ struct RhoDev {
   int d, kp, n;
   int nf[3];
   int hx;  // nf[0]/2 + 1
+  int cb;  // >= 0: S in the compact band layout of BandDft with band cb
   const double* psi0;
   const double* psi1;
   const double* psi2;
@@ -139,6 +142,7 @@ __global__ void k_nuwu_rho(const Cplx<T>* __restrict__ S, RhoDev r, int64_t m, d
   int hy = q[1] < 0 ? q[1] + r.nf[1] : q[1];
   int hz = q[2] < 0 ? q[2] + r.nf[2] : q[2];
   int64_t h = q[0] + (int64_t)r.hx * (hy + (int64_t)r.nf[1] * hz);
+  if (r.cb >= 0) h = ((int64_t)(q[2] + r.cb) * (2 * r.cb + 1) + (q[1] + r.cb)) * (r.cb + 1) + q[0];
   double ps = r.psi0[abs(k[0])];
   if (r.d >= 2) ps *= r.psi1[abs(k[1])];
   if (r.d >= 3) ps *= r.psi2[abs(k[2])];
@@ -255,7 +259,7 @@ class NUwU final : public Potential {
     spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
     {
       ProfScope p(ST_FFT_FWD, s);
-      r2c_.exec(grid_.get(), S_.get(), s);
+      forward_spectrum(s);
     }
     {
       ProfScope p(ST_LOSS, s);
@@ -298,6 +302,7 @@ class NUwU final : public Potential {
     r.n = nm_;
     for (int a = 0; a < 3; ++a) r.nf[a] = geom_.nf[a];
     r.hx = hx_;
+    r.cb = bd_ ? bdft_.band() : -1;
     r.psi0 = psi_[0].get();
     r.psi1 = psi_[1].get();
     r.psi2 = psi_[2].get();
@@ -360,11 +365,15 @@ class NUwU final : public Potential {
       pts_.set(x, map_g_, c_.get(), s);
     }
     spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
-    {
+    if (bd_) {
+      // the band transform's last pass also clears the pullback spectrum
       ProfScope p(ST_FFT_FWD, s);
-      r2c_.exec(grid_.get(), S_.get(), s);
-    }
-    {
+      bdft_.forward(grid_.get(), S_.get(), s, X_[0].get());
+    } else {
+      {
+        ProfScope p(ST_FFT_FWD, s);
+        forward_spectrum(s);
+      }
       ProfScope p(ST_ZERO, s);
       for (int a = 0; a < nX_; ++a) fill_zero(X_[a].get(), nspec_, s);
     }
@@ -388,11 +397,21 @@ class NUwU final : public Potential {
     } else {
       {
         ProfScope p(ST_FFT_INV, s);
-        c2r_.exec(X_[0].get(), grid_.get(), s);
+        if (bd_)
+          bdft_.inverse(X_[0].get(), grid_.get(), s);
+        else
+          c2r_.exec(X_[0].get(), grid_.get(), s);
       }
       ProfScope p(ST_INTERP, s);
       interpolate(pts_, grid_.get(), true, dscale_, grad, d_, 0, s);
     }
+  }
+
+  void forward_spectrum(hipStream_t s) {
+    if (bd_)
+      bdft_.forward(grid_.get(), S_.get(), s);
+    else
+      r2c_.exec(grid_.get(), S_.get(), s);
   }
 
   // largest |k_a| over modes whose pair {k, -k} carries weight; the FP64
@@ -425,13 +444,24 @@ class NUwU final : public Potential {
     spreader_.init(geom_, es_);
     grid_.allocate(geom_.ncells());
     hx_ = geom_.nf[0] / 2 + 1;
-    nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
+    // the adjoint path's 3D spectrum comes from pruned DFTs of the band when
+    // the band is small against the grid (FRESCO_BANDDFT=0: always the FFTs)
+    const char* env = std::getenv("FRESCO_BANDDFT");
+    bd_ = !plain_ && d_ == 3 && !(env && env[0] == '0') && BandDft<T>::suits(geom_.nf, band);
+    if (bd_) {
+      bdft_.init(geom_.nf, band);
+      nspec_ = bdft_.size();
+    } else {
+      nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
+    }
     S_.allocate(nspec_);
     nX_ = plain_ ? d_ : 1;
     for (int a = 0; a < nX_; ++a) X_[a].allocate(nspec_);
     const bool dbl = sizeof(T) == 8;
-    r2c_.create(FftPlan::R2C, dbl, d_, geom_.nf);
-    c2r_.create(FftPlan::C2R, dbl, d_, geom_.nf);
+    if (!bd_) {
+      r2c_.create(FftPlan::R2C, dbl, d_, geom_.nf);
+      c2r_.create(FftPlan::C2R, dbl, d_, geom_.nf);
+    }
     for (int a = 0; a < 3; ++a) {
       std::vector<double> ps = a < d_ ? es_deconv(es_, geom_.nf[a], kp_) : std::vector<double>(kp_ + 1, 1.0);
       psi_host_[a] = ps;
@@ -469,11 +499,12 @@ class NUwU final : public Potential {
       if (V[i] == 0.0 && V[j] == 0.0) continue;
       int hy = k[1] < 0 ? k[1] + geom_.nf[1] : k[1];
       int hz = k[2] < 0 ? k[2] + geom_.nf[2] : k[2];
-      h.push_back((int)(k[0] + (int64_t)hx_ * (hy + (int64_t)geom_.nf[1] * hz)));
+      h.push_back(bd_ ? (int)bdft_.index(k[0], k[1], k[2])
+                      : (int)(k[0] + (int64_t)hx_ * (hy + (int64_t)geom_.nf[1] * hz)));
       if (k[0] == 0 && j != i) {
         int ny = -k[1] < 0 ? -k[1] + geom_.nf[1] : -k[1];
         int nz = -k[2] < 0 ? -k[2] + geom_.nf[2] : -k[2];
-        hn.push_back((int)((int64_t)hx_ * (ny + (int64_t)geom_.nf[1] * nz)));
+        hn.push_back(bd_ ? (int)bdft_.index(0, -k[1], -k[2]) : (int)((int64_t)hx_ * (ny + (int64_t)geom_.nf[1] * nz)));
       } else {
         hn.push_back(-1);
       }
@@ -541,6 +572,8 @@ class NUwU final : public Potential {
   DeviceBuffer<Cplx<T>> X_[3];
   int nX_ = 1;
   FftPlan r2c_, c2r_;
+  bool bd_ = false;  // spectrum through bdft_ (compact band layout)
+  BandDft<T> bdft_;
   std::vector<double> psi_host_[3];
   DeviceBuffer<double> psi_[3];
   DeviceBuffer<int> p_h_, p_hn_, p_ik_, p_imk_, p_kv_;

@@ -258,6 +258,38 @@ def test_halo_spreader_wavefront_layouts_agree(NUwU, prec, eps, tol, monkeypatch
         assert np.array_equal(res["halo:" + sl][1], res["halo4:" + sl][1])
 
 
+@pytest.mark.parametrize("kmax", [6, 12])
+def test_band_dft_matches_fft_path(NUwU, kmax, monkeypatch):
+    """3D FP32 adjoint path with an active band |k_a| <= B small against the
+    fine grid: its spectrum comes from pruned separable DFTs of the band (six
+    twiddle-matrix passes) instead of the grid's real FFTs.  Same energy,
+    gradient and target derivatives as the FFT path (FRESCO_BANDDFT=0) to
+    FP32 rounding, and parity with the exact sums."""
+    rng = np.random.default_rng(50 + kmax)
+    n, N = 33, 4000
+    k = O.mode_grid(n // 2, 3)
+    kk = np.linalg.norm(k, axis=1)
+    V = np.where((kk > 0) & (kk <= kmax), rng.uniform(0.2, 1.0, len(kk)), 0.0)
+    Sk = np.where(rng.uniform(size=len(kk)) < 0.5, 0.0, rng.uniform(0.5, 2.0, len(kk)))
+    radii = rng.uniform(0.7, 1.3, N)
+    p = rng.uniform(0, 1, 3 * N)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FRESCO_BANDDFT", mode)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        E, g = pot.get_energy_gradient(p)
+        dS0, dV = pot.target_grads()
+        out[mode] = (E, g, dS0.cpu().numpy(), dV.cpu().numpy())
+    (E1, g1, s1, v1), (E0, g0, s0, v0) = out["1"], out["0"]
+    assert abs(E1 - E0) / abs(E0) < 2e-6
+    assert rel(g1, g0) < 2e-6
+    assert rel(s1, s0) < 2e-6 and rel(v1, v0) < 2e-6
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
+    Er, gr = ref.get_energy_gradient(p)
+    assert abs(E1 - Er) / abs(Er) < 1e-5
+    assert rel(g1, gr) < 1e-5
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64-adjoint"])
 def test_adjoint_gradient_with_weighted_k0(NUwU, prec):
     """V(k = 0) != 0 on a small 1D grid: the k = 0 term carries rho_0 = N but
