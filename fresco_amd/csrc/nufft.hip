@@ -2292,8 +2292,7 @@ __global__ void __launch_bounds__(G8_THREADS)
     int gx = t0[0] + lx;
     if (gx >= a.nf[0]) gx -= a.nf[0];
     if (gx >= a.nf[0]) gx -= a.nf[0];
-    const int nrows = (RY ? RY : a.ry) * a.rz;
-    for (int row = tid >> 5; row < nrows; row += G8_THREADS / 32) {
+    auto src = [&](int row) {
       const int ly = row % (RY ? RY : a.ry);
       const int lz = row / (RY ? RY : a.ry);
       int gy = t0[1] + ly;
@@ -2302,8 +2301,29 @@ __global__ void __launch_bounds__(G8_THREADS)
       int gz = t0[2] + lz;
       if (gz >= a.nf[2]) gz -= a.nf[2];
       if (gz >= a.nf[2]) gz -= a.nf[2];
-      if (lx < a.rx)
-        reg[(lz * (RY ? RY : a.ry) + ly) * (SRX ? SRX : a.srx) + lx] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+      return ((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx;
+    };
+    if constexpr (RY != 0) {
+      // fixed region (RY x 15 rows, the launcher checks rz): every load of
+      // a thread is issued before its LDS stores (one memory latency per
+      // workgroup instead of one per row)
+      constexpr int NROWS = RY * 15;
+      constexpr int PER = (NROWS + G8_THREADS / 32 - 1) / (G8_THREADS / 32);
+      T v[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int row = (tid >> 5) + q * (G8_THREADS / 32);
+        v[q] = (row < NROWS && lx < a.rx) ? grid[src(row)] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int row = (tid >> 5) + q * (G8_THREADS / 32);
+        if (row < NROWS && lx < a.rx) reg[row * SRX + lx] = v[q];
+      }
+    } else {
+      const int nrows = a.ry * a.rz;
+      for (int row = tid >> 5; row < nrows; row += G8_THREADS / 32)
+        if (lx < a.rx) reg[row * a.srx + lx] = grid[src(row)];
     }
   }
   __syncthreads();
@@ -2429,10 +2449,10 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   // enough workgroups to fill the GPU: split bins when there are few
   const int chunks = std::max(1, std::min(8, (2048 + g.nbins - 1) / g.nbins));
   const dim3 grd(g.nbins, chunks);
-  if (a.srx == 23 && a.ry == 15 && a.rx == 23)
+  if (a.srx == 23 && a.ry == 15 && a.rx == 23 && a.rz == 15)
     hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 23, 15>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
                        p.idx.get(), p.bin_start.get(), grid, a, out);
-  else if (a.srx == 23 && a.ry == 23 && a.rx == 23)
+  else if (a.srx == 23 && a.ry == 23 && a.rx == 23 && a.rz == 15)
     hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 23, 23>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
                        p.idx.get(), p.bin_start.get(), grid, a, out);
   else
