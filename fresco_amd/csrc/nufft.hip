@@ -2083,7 +2083,6 @@ static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
     hipLaunchKernelGGL(k_spread2_halo<T>, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64),
                        sizeof(T) * (size_t)helems, s, p.scell.get(), p.rec.get(), p.bin_start.get(), a, hs, hrows,
                        sp.halo_buf.get());
-#undef FRESCO_HALO_MW
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);
