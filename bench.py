@@ -66,7 +66,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box in this setup has one GPU; "
-              "multi-GPU sharding of the path is design-only (see DESIGN.md).", file=sys.stderr)
+              "multi-GPU sharding of the path is design-only.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -105,7 +105,7 @@ def test_sk_shaping_2d_fd(mods):
     gS = S0.grad.cpu().numpy()
 
     E64, g64 = p64.get_energy_gradient(p)
-    print("C3 energies fp32/fp64", E.item(), E64, flush=True)
+    print("2D N=2^18 energies fp32/fp64", E.item(), E64, flush=True)
     assert abs(E.item() - E64) / E64 < 1e-5
     assert rel(gx, g64 * TWO_PI) < 1e-5
 
@@ -122,7 +122,7 @@ def test_sk_shaping_2d_fd(mods):
     fd2 = np.array([dE(i, 5e-5) for i in idx])
     rich = (4 * fd2 - fd) / 3
     r1, r2 = rel(gx[idx], fd), rel(gx[idx], rich)
-    print("C3 x-grad vs FD h=1e-4:", r1, " Richardson:", r2, flush=True)
+    print("2D N=2^18 x-grad vs FD h=1e-4:", r1, " Richardson:", r2, flush=True)
     assert r1 < 2e-3  # central difference truncation O((2 pi k h)^2)
     assert r2 < 1e-5
 
@@ -143,7 +143,7 @@ def test_sk_shaping_2d_fd(mods):
         fdS.append((ep - em) / (2 * h))
     p64.set_targets(S0np, Vnp)
     rS = rel(gS[modes], np.array(fdS))
-    print("C3 S0-grad vs FD:", rS, flush=True)
+    print("2D N=2^18 S0-grad vs FD:", rS, flush=True)
     assert rS < 1e-5
 
 

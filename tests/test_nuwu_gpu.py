@@ -151,7 +151,7 @@ def test_fd_directional_derivative_fp64(NUwU):
     assert abs(fd - 2 * math.pi * np.dot(g, v)) / abs(fd) < 1e-6
 
 
-def test_c2_size_fp32_vs_fp64_and_invariants(NUwU):
+def test_3d_n2p20_fp32_vs_fp64_and_invariants(NUwU):
     """Stealthy |k| <= 24 on 129^3 modes, N = 2^20: FP32-adjoint vs FP64 plain
     (energy and gradient within 1e-5), Sum(grad) = 0 and translation invariance
     of the energy."""
@@ -169,10 +169,10 @@ def test_c2_size_fp32_vs_fp64_and_invariants(NUwU):
     xt = torch.tensor(p, device="cuda")
     E64, g64 = p64.get_energy_gradient(xt)
     E32, g32 = p32.get_energy_gradient(xt)
-    print("C2 energies", E64, E32, flush=True)
+    print("3D N=2^20 energies", E64, E32, flush=True)
     assert abs(E32 - E64) / abs(E64) < 1e-5
     r = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
-    print("C2 gradient rel err fp32 vs fp64", r, flush=True)
+    print("3D N=2^20 gradient rel err fp32 vs fp64", r, flush=True)
     assert r < 1e-5
     gs = g64.reshape(-1, 3).sum(0).abs().max().item()
     assert gs < 1e-8 * float(g64.abs().max()) * N ** 0.5

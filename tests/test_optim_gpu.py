@@ -150,7 +150,7 @@ def test_combined_potential_sums(mods):
     assert np.isclose(c.get_energy(x), Ea + Eb, rtol=1e-12)
 
 
-def test_c1_config_lbfgs_teacher_forced(mods):
+def test_2d_n4096_lbfgs_teacher_forced(mods):
     """2D NUwU, N=4096, 65^2 modes, stealthy 0<|k|<=16, eps 1e-9, LBFGS M=10:
     every 50 iterations the device energy / gradient at the device iterate
     equal the oracle's exact direct sum at that iterate."""
@@ -171,7 +171,7 @@ def test_c1_config_lbfgs_teacher_forced(mods):
     while True:
         res = opt.run(20)
         E, g = ref.get_energy_gradient(res.coords)
-        print("C1 it", res.nsteps, "E", res.energy, "oracle", E, "rms", res.rms, flush=True)
+        print("2D N=4096 it", res.nsteps, "E", res.energy, "oracle", E, "rms", res.rms, flush=True)
         assert abs(res.energy - E) <= 1e-8 * abs(E0)
         assert np.linalg.norm(res.grad - g) <= 1e-8 * np.linalg.norm(g0)
         if res.success or res.nsteps >= 2000:
@@ -180,7 +180,7 @@ def test_c1_config_lbfgs_teacher_forced(mods):
     assert np.linalg.norm(g) < 1e-7
 
 
-def test_c4_config_nuwnu_type3_lbfgs(mods):
+def test_2d_nuwnu_n2p16_type3_lbfgs(mods):
     """NUwNU 2D, N = 2^16 points, 2^16 k uniform in |k| <= 128 (S0 = 0, V = 1)
     plus 32 peaks |k| in [20, 100] (S0 = N, V = 1): FP32 type-3 NUFFT against
     the exact FP64 sums (energy / gradient 1e-5 at the start and after LBFGS),
@@ -212,13 +212,13 @@ def test_c4_config_nuwnu_type3_lbfgs(mods):
     compare(x0, abs(E0), np.linalg.norm(g0))
     opt = LBFGS_CPP(x0, p32, M=10, nsteps=500, tol=0.0)
     res = opt.run(500)
-    print("C4 LBFGS", res.nsteps, "E", E0, "->", res.energy, flush=True)
+    print("NUwNU 2D N=2^16 LBFGS", res.nsteps, "E", E0, "->", res.energy, flush=True)
     assert res.nsteps == 500
     assert res.energy < 0.8 * E0  # measured: 1.80e9 -> 1.21e9
     compare(res.coords, abs(E0), np.linalg.norm(g0))
 
 
-def test_c2_config_lbfgs_teacher_forced(mods):
+def test_3d_n2p20_lbfgs_teacher_forced(mods):
     """3D NUwU, N = 2^20, 129^3 modes, stealthy 0 < |k| <= 24: 200 LBFGS
     iterations (M = 4) on the FP64 plain path; every 25 iterations the FP32
     adjoint path is evaluated at the FP64 iterate (teacher forcing).  The
@@ -261,7 +261,7 @@ def test_c2_config_lbfgs_teacher_forced(mods):
         dg = float(torch.linalg.norm(g32 - g64))
         ds = float(np.linalg.norm(s32 - s64))
         rel = (de / abs(e64), dg / float(torch.linalg.norm(g64)), ds / np.linalg.norm(s64))
-        print("C2 it", res.nsteps, "E", e64, "rel E, g, S", rel, "vs x0 scale", de / e0, dg / gn0, ds / sn0,
+        print("3D N=2^20 it", res.nsteps, "E", e64, "rel E, g, S", rel, "vs x0 scale", de / e0, dg / gn0, ds / sn0,
               flush=True)
         assert de <= 1e-5 * e0 and dg <= 1e-5 * gn0 and ds <= 1e-5 * sn0
         if e64 > 1e-3 * e0:
