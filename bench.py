@@ -65,7 +65,7 @@ def main():
                     help="also run a plain 256 MiB device copy (the HBM byte-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box in this setup has one GPU; "
+        print("bench.py measures one GPU only (the N = 1 throughput); "
               "multi-GPU sharding of the path is design-only.", file=sys.stderr)
         sys.exit(2)
 
