@@ -27,9 +27,8 @@ class BandDft {
  public:
   // nf: fine grid (x fastest); band: B
   void init(const int* nf, int band);
-  bool ready() const { return B_ >= 0; }
-  // largest band this path takes for a grid of n cells per dimension (the
-  // passes cost ~ n^3 B; beyond it the FFTs are cheaper)
+  // bands and grids this path takes (the passes cost ~ n^3 B against the
+  // FFT's n^3 log n; contracted axes up to 128 cells)
   static bool suits(const int* nf, int band) {
     return band >= 0 && band <= 32 && nf[0] <= 128 && nf[1] <= 128 && nf[2] <= 128;
   }
