@@ -533,7 +533,9 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   scell_in.allocate(n);
   rec_in.allocate(n);
   key.allocate(n);
-  direct = tb <= PB_MAXB;
+  // FRESCO_DIRECT_BINNING=0: always the radix sort (the same order)
+  const char* env = std::getenv("FRESCO_DIRECT_BINNING");
+  direct = tb <= PB_MAXB && !(env && env[0] == '0');
   if (direct) {
     ntiles_ = (int)((n + PB_TILE - 1) / PB_TILE);
     gsz_ = 16 * (int)((ntiles_ + 16 * PB_GMAX - 1) / (16 * PB_GMAX));

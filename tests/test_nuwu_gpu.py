@@ -258,6 +258,23 @@ def test_halo_spreader_wavefront_layouts_agree(NUwU, prec, eps, tol, monkeypatch
         assert np.array_equal(res["halo:" + sl][1], res["halo4:" + sl][1])
 
 
+@pytest.mark.parametrize("d,n,N", [(3, 33, 5000), (3, 21, 2047), (2, 65, 4097), (1, 129, 3000)])
+def test_direct_binning_equals_radix_sort(NUwU, d, n, N, monkeypatch):
+    """Plans whose bins fit one 11-bit digit are binned by one counting pass
+    (tile counts, group prefixes, ballot-ranked placement) instead of the
+    radix sort; both give index order inside every bin, so energy and
+    gradient are bitwise equal.  Sizes off the 2048-point tile."""
+    radii, Sk, V, p = make_case(d, n, N, seed=60 + d)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FRESCO_DIRECT_BINNING", mode)
+        monkeypatch.setenv("FRESCO_SPREAD3", "halo4:2")
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(d), 1e-6, precision="fp32")
+        out[mode] = pot.get_energy_gradient(p)
+    assert out["1"][0] == out["0"][0]
+    assert np.array_equal(out["1"][1], out["0"][1])
+
+
 @pytest.mark.parametrize("kmax", [6, 12])
 def test_band_dft_matches_fft_path(NUwU, kmax, monkeypatch):
     """3D FP32 adjoint path with an active band |k_a| <= B small against the
