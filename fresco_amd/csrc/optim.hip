@@ -17,6 +17,7 @@
 #include <iostream>
 #include <limits>
 #include <memory>
+#include <utility>
 
 #include "optim.hpp"
 
@@ -1298,7 +1299,10 @@ void ModifiedFire::one_iteration() {
   iter_number_ += 1;
   fire_iter_number_ += 1;
   fold_ = f_;
-  ops_.copy(xold_.get(), x_.get(), stream_);
+  // x_old = x, g_old = g (modified_fire.hpp:146, 168) by exchanging buffer
+  // ownership instead of copying: the step below writes the new positions
+  // into the other buffer, the evaluation the new gradient
+  std::swap(x_, xold_);
   // velocity mixing + forward Euler (modified_fire.hpp:136-158, 172-177)
   unsigned nb = nblocks(n_);
   hipLaunchKernelGGL(k_fire_euler, dim3(nb), dim3(RB), 0, stream_, v_.get(), g_.get(), dx_.get(), a_,
@@ -1306,13 +1310,13 @@ void ModifiedFire::one_iteration() {
   hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(RB), 0, stream_, ops_.part.get(), (int)nb, 1, 1.0, nullptr,
                      ops_.scal.get());
   FRESCO_HIP_CHECK(hipGetLastError());
-  ops_.copy(gold_.get(), g_.get(), stream_);
+  std::swap(g_, gold_);
   double nd;
   ops_.fetch(1, &nd, stream_);
   double normdx = std::sqrt(nd);
   double factor = 1;
   if (normdx > fmaxstep_) factor = fmaxstep_ / normdx;
-  ops_.axpbz(x_.get(), 1.0, x_.get(), factor, dx_.get(), stream_);
+  ops_.axpbz(x_.get(), 1.0, xold_.get(), factor, dx_.get(), stream_);
   f_ = pot_->energy_gradient(x_.get(), g_.get(), stream_);
   double r[3];
   ops_.dots3(v_.get(), g_.get(), g_.get(), g_.get(), v_.get(), v_.get(), r, stream_);
@@ -1331,9 +1335,10 @@ void ModifiedFire::one_iteration() {
     fire_iter_number_ = 0;
     ops_.zero(v_.get(), stream_);
     if (stepback_) {
+      // x = x_old, g = g_old (modified_fire.hpp:207-211): back to the old buffers
       f_ = fold_;
-      ops_.copy(x_.get(), xold_.get(), stream_);
-      ops_.copy(g_.get(), gold_.get(), stream_);
+      std::swap(x_, xold_);
+      std::swap(g_, gold_);
     }
     if (verbosity_ > 1) std::cout << "warning: step direction was uphill.  inverting\n";
   }
