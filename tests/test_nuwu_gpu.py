@@ -307,6 +307,36 @@ def test_band_dft_matches_fft_path(NUwU, kmax, monkeypatch):
     assert rel(g1, gr) < 1e-5
 
 
+@pytest.mark.parametrize("band", [0, 1, None])
+def test_band_dft_smallest_bands(NUwU, band, monkeypatch):
+    """The band transform at its smallest: weight only on k = 0 (band 0), on
+    |k_a| <= 1 (band 1), and on no mode at all (energy and gradient 0);
+    parity with the exact sums and with the FFT path."""
+    rng = np.random.default_rng(70)
+    n, N = 9, 500
+    k = O.mode_grid(n // 2, 3)
+    V = np.zeros(len(k))
+    if band is not None:
+        V[np.max(np.abs(k), axis=1) <= band] = rng.uniform(0.5, 1.0, int(np.sum(np.max(np.abs(k), axis=1) <= band)))
+    Sk = rng.uniform(0.0, 2.0, len(k))
+    radii = rng.uniform(0.8, 1.2, N)
+    p = rng.uniform(0, 1, 3 * N)
+    res = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FRESCO_BANDDFT", mode)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        res[mode] = pot.get_energy_gradient(p)
+    E, g = res["1"]
+    if band is None:
+        assert E == 0.0 and not np.any(g)
+        return
+    Er, gr = O.NUwU(radii, 0.0, Sk, V, np.ones(3)).get_energy_gradient(p)
+    assert abs(E - Er) / abs(Er) < 1e-5
+    assert abs(E - res["0"][0]) / abs(Er) < 2e-6
+    gn = max(np.linalg.norm(gr), 1e-300)
+    assert np.linalg.norm(g - gr) / gn < 1e-5 or np.linalg.norm(g - gr) < 1e-5 * abs(Er)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64-adjoint"])
 def test_adjoint_gradient_with_weighted_k0(NUwU, prec):
     """V(k = 0) != 0 on a small 1D grid: the k = 0 term carries rho_0 = N but
