@@ -9,6 +9,7 @@
 #include "banddft.hpp"
 
 #include <cmath>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -155,7 +156,8 @@ void launch_pass(const void* in, int64_t so, int64_t si, int64_t sj, int O, int 
   constexpr int ITILE = LANE_K ? 8 * NPT : 32;
   constexpr int KTILE = LANE_K ? 32 : 8 * NPT;
   const size_t lds = (((size_t)J * ITILE * sizeof(IT) + 15) & ~(size_t)15) + (size_t)KTILE * (J | 1) * sizeof(Cplx<T>);
-  if (lds > 160 * 1024 || J > BD_JMAX) throw InvalidArgument("band transform pass exceeds LDS");
+  if (lds > 160 * 1024 || J > BD_JMAX)
+    throw InvalidArgument("band transform pass: contracted axis over " + std::to_string(BD_JMAX) + " or LDS exceeded");
   const dim3 grd(O, (I + ITILE - 1) / ITILE, (K + KTILE - 1) / KTILE);
   hipLaunchKernelGGL((k_band_pass<T, IN_REAL, OUT_REAL, LANE_K, NPT>), grd, dim3(BD_T), lds, s, in, so, si, sj, I, J,
                      W, K, out, to, ti, tk, zero_out);
