@@ -114,6 +114,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_int, c_int64_p, c_double_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "fresco_radial_profile_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int, c_int64_p, c_double_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "fresco_opt_get_g": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),

@@ -432,6 +432,14 @@ int fresco_radial_profile(const double* data, int d, const int64_t* shape, const
   });
 }
 
+int fresco_radial_profile_noalloc(const double* data, int d, const int64_t* shape, const double* center,
+                                  double* sums, int64_t* counts, void* stream) {
+  return guarded([&]() {
+    if (!shape || !center) throw fresco_hip::InvalidArgument("null pointer");
+    fresco_hip::radial_profile_noalloc(data, d, shape, center, sums, counts, static_cast<hipStream_t>(stream));
+  });
+}
+
 int fresco_pot_destroy(fresco_pot* pot) {
   return guarded([&]() { delete pot; });
 }

@@ -36,7 +36,8 @@ bool consume_fault();  // true if an injected allocation failure is pending
     }                                                                               \
   } while (0)
 
-// Owning device allocation (hipMalloc'd, freed on destruction).
+// Device allocation: owning (hipMalloc'd by allocate, freed on destruction)
+// or borrowed (borrow: a view of memory someone else owns, never freed).
 template <typename T>
 class DeviceBuffer {
  public:
@@ -45,7 +46,7 @@ class DeviceBuffer {
   ~DeviceBuffer() { release(); }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), n_(o.n_) {
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), n_(o.n_), owned_(o.owned_) {
     o.ptr_ = nullptr;
     o.n_ = 0;
   }
@@ -54,6 +55,7 @@ class DeviceBuffer {
       release();
       ptr_ = o.ptr_;
       n_ = o.n_;
+      owned_ = o.owned_;
       o.ptr_ = nullptr;
       o.n_ = 0;
     }
@@ -67,11 +69,20 @@ class DeviceBuffer {
     FRESCO_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
     ptr_ = static_cast<T*>(p);
     n_ = n;
+    owned_ = true;
+  }
+  // n elements at p (device memory the caller keeps alive); no allocation
+  void borrow(T* p, size_t n) {
+    release();
+    ptr_ = n ? p : nullptr;
+    n_ = n;
+    owned_ = false;
   }
   void release() {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_ && owned_) (void)hipFree(ptr_);
     ptr_ = nullptr;
     n_ = 0;
+    owned_ = false;
   }
   void upload(const T* host, size_t n, hipStream_t s = nullptr) {
     if (n > n_) throw RuntimeFailure("upload larger than buffer");
@@ -85,6 +96,31 @@ class DeviceBuffer {
  private:
   T* ptr_ = nullptr;
   size_t n_ = 0;
+  bool owned_ = false;
+};
+
+// Carves 256-byte-aligned pieces out of one caller-owned device region.
+class Carve {
+ public:
+  Carve(void* base, size_t bytes) : p_(static_cast<char*>(base)), left_(bytes) {}
+  static size_t rounded(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+  template <typename T>
+  T* take(size_t n) {
+    const size_t b = rounded(n * sizeof(T));
+    if (b > left_) throw InvalidArgument("workspace too small");
+    T* r = reinterpret_cast<T*>(p_);
+    p_ += b;
+    left_ -= b;
+    return r;
+  }
+  template <typename T>
+  void take(DeviceBuffer<T>& buf, size_t n) {
+    buf.borrow(take<T>(n), n);
+  }
+
+ private:
+  char* p_;
+  size_t left_;
 };
 
 inline unsigned blocks_for(size_t n, unsigned threads) {

@@ -349,19 +349,62 @@ __global__ void __launch_bounds__(RS_T)
 
 }  // namespace
 
-void BinSort::init(int64_t n, int key_bits) {
+namespace {
+struct SortShape {
+  int passes, dbits, nblk, nseg;
+  int64_t m;
+};
+
+SortShape sort_shape(int64_t n, int key_bits) {
   if (n <= 0 || n >= (1LL << 31)) throw InvalidArgument("radix sort size out of range");
-  n_ = n;
   key_bits = std::max(1, key_bits);
-  passes_ = (key_bits + 10) / 11;
-  dbits_ = (key_bits + passes_ - 1) / passes_;
-  nblk_ = (int)((n + RS_TILE - 1) / RS_TILE);
-  const int64_t m = ((int64_t)1 << dbits_) * nblk_;
-  nseg_ = (int)((m + RS_SEG - 1) / RS_SEG);
-  cnt_.allocate((size_t)m);
+  SortShape h;
+  h.passes = (key_bits + 10) / 11;
+  h.dbits = (key_bits + h.passes - 1) / h.passes;
+  h.nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+  h.m = ((int64_t)1 << h.dbits) * h.nblk;
+  h.nseg = (int)((h.m + RS_SEG - 1) / RS_SEG);
+  return h;
+}
+}  // namespace
+
+void BinSort::init(int64_t n, int key_bits) {
+  const SortShape h = sort_shape(n, key_bits);
+  n_ = n;
+  passes_ = h.passes;
+  dbits_ = h.dbits;
+  nblk_ = h.nblk;
+  nseg_ = h.nseg;
+  cnt_.allocate((size_t)h.m);
   tops_.allocate((size_t)nseg_);
   ktmp_.allocate(n);
   vtmp_.allocate(n);
+}
+
+size_t BinSort::workspace_bytes(int64_t n, int key_bits) {
+  const SortShape h = sort_shape(n, key_bits);
+  size_t b = Carve::rounded((size_t)h.m * 4) + Carve::rounded((size_t)h.nseg * 4);
+  if (h.passes > 1) b += 2 * Carve::rounded((size_t)n * 4);
+  return b;
+}
+
+void BinSort::init(int64_t n, int key_bits, Carve& ws) {
+  const SortShape h = sort_shape(n, key_bits);
+  n_ = n;
+  passes_ = h.passes;
+  dbits_ = h.dbits;
+  nblk_ = h.nblk;
+  nseg_ = h.nseg;
+  ws.take(cnt_, (size_t)h.m);
+  ws.take(tops_, (size_t)nseg_);
+  // single-pass sorts write the outputs directly: no ping-pong buffers
+  if (passes_ > 1) {
+    ws.take(ktmp_, (size_t)n);
+    ws.take(vtmp_, (size_t)n);
+  } else {
+    ktmp_.release();
+    vtmp_.release();
+  }
 }
 
 void BinSort::set_segments(const std::vector<int64_t>& starts, int nbuck) {

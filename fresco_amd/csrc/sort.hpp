@@ -26,6 +26,11 @@ class BinSort {
  public:
   // n pairs; keys are below 2^key_bits
   void init(int64_t n, int key_bits);
+  // the same, with the count tables (and the ping-pong buffers of
+  // multi-pass sorts) carved from ws instead of allocated
+  void init(int64_t n, int key_bits, Carve& ws);
+  // bytes init(n, key_bits, ws) takes from ws
+  static size_t workspace_bytes(int64_t n, int key_bits);
   // keys / vals -> keys_out / vals_out (stable; inputs are not modified);
   // with a payload, its records are placed in the output order too
   void sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s,

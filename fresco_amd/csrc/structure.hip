@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "nufft.hpp"
@@ -320,13 +321,15 @@ void structure_factor(const double* x, const double* c, int64_t n, int d, const 
     structure_factor_t<double>(x, c, n, d, L, (int)nk, eps, Sk, gr, s);
 }
 
-void radial_profile(const double* data, int d, const int64_t* shape, const double* center, double* sums,
-                    int64_t* counts, hipStream_t s) {
+namespace {
+
+RpShape rp_shape(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                 int64_t* counts, int64_t& n) {
   if (d < 1 || d > 3) throw InvalidArgument("radial profiles take 1 to 3 dimensions");
   if (!data || !sums || !counts) throw InvalidArgument("null pointer");
   RpShape sh;
   sh.d = d;
-  int64_t n = 1;
+  n = 1;
   for (int a = 0; a < 3; ++a) {
     sh.s[a] = a < d ? shape[a] : 1;
     sh.c[a] = a < d ? center[a] : 0.0;
@@ -335,34 +338,122 @@ void radial_profile(const double* data, int d, const int64_t* shape, const doubl
   }
   if (n >= (1LL << 31)) throw InvalidArgument("grid too large for the shell sort (2^31 cells)");
   sh.R = (int)(shape[0] / 2);
-  const int nb = sh.R + 2;
+  return sh;
+}
+
+// Device buffers of one radial profile: n cells, nb shells.
+struct RpBuffers {
   DeviceBuffer<uint32_t> key, val, ks, vs;
-  key.allocate(n);
-  val.allocate(n);
-  ks.allocate(n);
-  vs.allocate(n);
   DeviceBuffer<int64_t> start, end, cpre;
-  start.allocate(nb);
-  end.allocate(nb);
-  cpre.allocate(nb + 1);
-  const int64_t maxchunks = (n + RP_CHUNK - 1) / RP_CHUNK + nb;
   DeviceBuffer<double> part;
-  part.allocate(maxchunks);
   BinSort sorter;
-  sorter.init(n, std::max(1, bits_for(nb)));
-  hipLaunchKernelGGL(k_rp_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, sh, n, key.get(), val.get());
-  hipLaunchKernelGGL(k_rp_clear, dim3((nb + 255) / 256), dim3(256), 0, s, start.get(), nb);
-  hipLaunchKernelGGL(k_rp_clear, dim3((nb + 255) / 256), dim3(256), 0, s, end.get(), nb);
+};
+
+int64_t rp_max_chunks(int64_t n, int nb) { return (n + RP_CHUNK - 1) / RP_CHUNK + nb; }
+
+// shell keys, stable sort by shell, fixed-chunk partial sums, per-shell sums:
+// every buffer is written before it is read (no fills)
+void rp_run(const RpShape& sh, int64_t n, const double* data, double* sums, int64_t* counts, RpBuffers& b,
+            hipStream_t s) {
+  const int nb = sh.R + 2;
+  const int64_t maxchunks = rp_max_chunks(n, nb);
+  hipLaunchKernelGGL(k_rp_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, sh, n, b.key.get(), b.val.get());
+  hipLaunchKernelGGL(k_rp_clear, dim3((nb + 255) / 256), dim3(256), 0, s, b.start.get(), nb);
+  hipLaunchKernelGGL(k_rp_clear, dim3((nb + 255) / 256), dim3(256), 0, s, b.end.get(), nb);
   FRESCO_HIP_CHECK(hipGetLastError());
-  sorter.sort(key.get(), val.get(), ks.get(), vs.get(), s);
-  hipLaunchKernelGGL(k_rp_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, ks.get(), n, start.get(), end.get());
-  hipLaunchKernelGGL(k_rp_chunks, dim3(1), dim3(64), 0, s, start.get(), end.get(), nb, cpre.get());
-  hipLaunchKernelGGL(k_rp_partial, dim3(blocks_for(maxchunks, 256)), dim3(256), 0, s, vs.get(), data, start.get(),
-                     end.get(), cpre.get(), nb, part.get());
-  hipLaunchKernelGGL(k_rp_final, dim3((nb + 255) / 256), dim3(256), 0, s, cpre.get(), part.get(), start.get(),
-                     end.get(), nb, sums, counts);
+  b.sorter.sort(b.key.get(), b.val.get(), b.ks.get(), b.vs.get(), s);
+  hipLaunchKernelGGL(k_rp_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.ks.get(), n, b.start.get(),
+                     b.end.get());
+  hipLaunchKernelGGL(k_rp_chunks, dim3(1), dim3(64), 0, s, b.start.get(), b.end.get(), nb, b.cpre.get());
+  hipLaunchKernelGGL(k_rp_partial, dim3(blocks_for(maxchunks, 256)), dim3(256), 0, s, b.vs.get(), data,
+                     b.start.get(), b.end.get(), b.cpre.get(), nb, b.part.get());
+  hipLaunchKernelGGL(k_rp_final, dim3((nb + 255) / 256), dim3(256), 0, s, b.cpre.get(), b.part.get(), b.start.get(),
+                     b.end.get(), nb, sums, counts);
   FRESCO_HIP_CHECK(hipGetLastError());
   FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Scratch of the allocation-free radial profile: one array per GPU, held by
+// the loaded code object until the process ends.  Its address on the current
+// device comes from hipGetSymbolAddress (the name is only a host-side stand-in).
+constexpr size_t RP_SCRATCH_BYTES = (size_t)20 << 20;
+constexpr size_t rp_round(size_t b) { return (b + 255) & ~(size_t)255; }
+// largest carve: 4 key/index arrays, shell bounds and chunk prefix, chunk
+// partials, and the one-pass sort's count table (2^11 digits x 4096-key
+// tiles) with its segment totals
+static_assert(4 * rp_round(RP_NOALLOC_MAX_CELLS * 4) + 2 * rp_round(RP_NOALLOC_MAX_SHELLS * 8) +
+                      rp_round((RP_NOALLOC_MAX_SHELLS + 1) * 8) +
+                      rp_round(((RP_NOALLOC_MAX_CELLS + RP_CHUNK - 1) / RP_CHUNK + RP_NOALLOC_MAX_SHELLS) * 8) +
+                      rp_round(2048 * ((RP_NOALLOC_MAX_CELLS + 4095) / 4096) * 4) +
+                      rp_round((2048 * ((RP_NOALLOC_MAX_CELLS + 4095) / 4096) + 4095) / 4096 * 4) <=
+                  RP_SCRATCH_BYTES,
+              "radial profile scratch too small for its limits");
+static __device__ __align__(256) unsigned char scratch[RP_SCRATCH_BYTES];
+
+constexpr int RP_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[RP_MAX_DEVICES] = {};
+
+void* rp_scratch_on_current_device() {
+  int dev = 0;
+  FRESCO_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= RP_MAX_DEVICES) throw RuntimeFailure("device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw HipFailure(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+void radial_profile(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                    int64_t* counts, hipStream_t s) {
+  int64_t n = 0;
+  const RpShape sh = rp_shape(data, d, shape, center, sums, counts, n);
+  const int nb = sh.R + 2;
+  RpBuffers b;
+  b.key.allocate(n);
+  b.val.allocate(n);
+  b.ks.allocate(n);
+  b.vs.allocate(n);
+  b.start.allocate(nb);
+  b.end.allocate(nb);
+  b.cpre.allocate(nb + 1);
+  b.part.allocate(rp_max_chunks(n, nb));
+  b.sorter.init(n, std::max(1, bits_for(nb)));
+  rp_run(sh, n, data, sums, counts, b, s);
+}
+
+void radial_profile_noalloc(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                            int64_t* counts, hipStream_t s) {
+  int64_t n = 0;
+  const RpShape sh = rp_shape(data, d, shape, center, sums, counts, n);
+  const int nb = sh.R + 2;
+  if (n > RP_NOALLOC_MAX_CELLS)
+    throw InvalidArgument("radial_profile_noalloc: " + std::to_string(n) + " cells exceed its limit of " +
+                          std::to_string(RP_NOALLOC_MAX_CELLS));
+  if (nb > RP_NOALLOC_MAX_SHELLS)
+    throw InvalidArgument("radial_profile_noalloc: shape[0] = " + std::to_string(shape[0]) + " gives " +
+                          std::to_string(nb) + " shells, beyond its limit of " +
+                          std::to_string(RP_NOALLOC_MAX_SHELLS) + " (shape[0] <= " +
+                          std::to_string(2 * (RP_NOALLOC_MAX_SHELLS - 2) + 1) + ")");
+  Carve ws(rp_scratch_on_current_device(), RP_SCRATCH_BYTES);
+  RpBuffers b;
+  ws.take(b.key, (size_t)n);
+  ws.take(b.val, (size_t)n);
+  ws.take(b.ks, (size_t)n);
+  ws.take(b.vs, (size_t)n);
+  ws.take(b.start, (size_t)nb);
+  ws.take(b.end, (size_t)nb);
+  ws.take(b.cpre, (size_t)nb + 1);
+  ws.take(b.part, (size_t)rp_max_chunks(n, nb));
+  b.sorter.init(n, std::max(1, bits_for(nb)), ws);
+  rp_run(sh, n, data, sums, counts, b, s);
 }
 
 }  // namespace fresco_hip

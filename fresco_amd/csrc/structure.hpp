@@ -26,4 +26,17 @@ void structure_factor(const double* x, const double* c, int64_t n, int d, const 
 void radial_profile(const double* data, int d, const int64_t* shape, const double* center, double* sums,
                     int64_t* counts, hipStream_t s);
 
+// Limits of radial_profile_noalloc: cells of the grid, and shells
+// (shape[0] / 2 + 2, so that the shell keys sort in one 11-bit pass).
+constexpr int64_t RP_NOALLOC_MAX_CELLS = (int64_t)1 << 20;
+constexpr int RP_NOALLOC_MAX_SHELLS = 2048;
+
+// radial_profile without any device allocation: its buffers live in a fixed
+// per-GPU scratch array of the library.  Same kernels and summation order,
+// so the results equal radial_profile's.  Grids beyond the limits above are
+// rejected (InvalidArgument).  Calls that overlap on one GPU race on the
+// scratch array.  Synchronises s before returning.
+void radial_profile_noalloc(const double* data, int d, const int64_t* shape, const double* center, double* sums,
+                            int64_t* counts, hipStream_t s);
+
 }  // namespace fresco_hip

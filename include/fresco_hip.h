@@ -285,6 +285,24 @@ int fresco_radial_profile(const double* data, int d, const int64_t* shape, const
                           int64_t* counts, void* stream);
 
 /*
+ * fresco_radial_profile without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same arguments, kernels and summation order as fresco_radial_profile, so
+ * the results are equal; its working buffers live in a fixed scratch array
+ * of the library (20 MiB per GPU, held until the process ends).
+ * Limits: at most FRESCO_RP_NOALLOC_MAX_CELLS cells and shape[0] <=
+ * FRESCO_RP_NOALLOC_MAX_SHAPE0; larger grids return FRESCO_E_INVALID.
+ * Every call on one GPU shares that scratch: calls that overlap on one GPU
+ * (other host threads, other streams) race.  Returns after the results are
+ * complete.
+ */
+#define FRESCO_RP_NOALLOC_MAX_CELLS 1048576
+#define FRESCO_RP_NOALLOC_MAX_SHAPE0 4093
+int fresco_radial_profile_noalloc(const double* data, int d, const int64_t* shape, const double* center,
+                                  double* sums, int64_t* counts, void* stream);
+
+/*
  * Sum of potentials over the same coordinates.
  * Replaces fresco::CombinedPotential (source/fresco/combine_potentials.hpp:16-59),
  * fresco/potentials/combine_potentials.pyx:10-30.  The children stay alive while
