@@ -1841,6 +1841,10 @@ __global__ void __launch_bounds__(64 * NW)
         for (int m = 0; m < NP; ++m) zv[h][m] = wzs[pp][j1 + m * NW];
       }
     }
+    // Every LDS load above completes here, so the point loop below waits
+    // once per point (for its own reads) instead of also waiting for the
+    // previous point's write at a merged loop-entry state.
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
 #pragma unroll
     for (int h = 0; h * 64 < CH; ++h) {
       const int lim = min(cnt - h * 64, 64);
