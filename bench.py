@@ -196,7 +196,8 @@ def main():
             "nf": nf,
             "w": info32["w"],
             "tile": info32["tile"],
-            "spreader": "%s, %d slices/bin (autotuned)" % (info32["spreader"], info32["spread_parts"]),
+            "spreader": "%s, %d slices/bin (%s)" % (info32["spreader"], info32["spread_parts"],
+                                                     info32["spread_source"]),
         },
         "points_per_s": round(ev32 * N, 1),
         "fp64_plain": {

@@ -74,6 +74,7 @@ SIGNATURES = {
     "fresco_pot_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_set_targets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_spread_config": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
+    "fresco_pot_spread_source": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "fresco_pot_ndof": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_nmodes": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "fresco_pot_plan_info": (

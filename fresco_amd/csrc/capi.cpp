@@ -382,6 +382,13 @@ int fresco_pot_spread_config(fresco_pot* pot, int* mode, int* parts) {
   });
 }
 
+int fresco_pot_spread_source(fresco_pot* pot, int* source) {
+  return guarded([&]() {
+    if (!source) throw fresco_hip::InvalidArgument("null pointer");
+    *source = checked(pot)->spread_source();
+  });
+}
+
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof) {
   return guarded([&]() { *ndof = checked(pot)->ndof(); });
 }

@@ -3,10 +3,16 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <dlfcn.h>
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
-#include <mutex>
+#include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
 #include <string>
 
 #include "nufft.hpp"
@@ -1044,15 +1050,114 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
         if (v.size() > 5) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 5)));
       }
       tuned = true;
+      tune_source = 3;
     }
   }
 }
 
+// ---------------------------------------------------------------------------
+// autotuning cache: the 3D spreader picked for a plan shape, so that separate
+// processes spread with the same kernel (and round identically).  Entries
+// are lines "<key> <mode> <slices>"; the first line of a key wins.  Lookup:
+// the user cache ($FRESCO_AUTOTUNE_CACHE, else
+// $XDG_CACHE_HOME/fresco_amd/spread3.txt or ~/.cache/fresco_amd/spread3.txt;
+// FRESCO_AUTOTUNE_CACHE=off disables it), then the table shipped beside the
+// library (tuning/spread3_mi355x.txt).  New picks go to the user cache.
+// ---------------------------------------------------------------------------
+
+namespace {
+std::mutex g_tune_mu;
+
+std::string user_tune_path() {
+  if (const char* e = std::getenv("FRESCO_AUTOTUNE_CACHE")) {
+    const std::string v(e);
+    if (v.empty() || v == "off" || v == "0") return "";
+    return v;
+  }
+  std::string dir;
+  if (const char* x = std::getenv("XDG_CACHE_HOME")) dir = std::string(x);
+  else if (const char* h = std::getenv("HOME")) dir = std::string(h) + "/.cache";
+  if (dir.empty()) return "";
+  return dir + "/fresco_amd/spread3.txt";
+}
+
+std::string shipped_tune_path() {
+  Dl_info info;
+  if (!dladdr(reinterpret_cast<void*>(&shipped_tune_path), &info) || !info.dli_fname) return "";
+  std::string so(info.dli_fname);
+  const size_t slash = so.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning/spread3_mi355x.txt";
+}
+
+bool tune_lookup(const std::string& path, const std::string& key, int* mode, int* parts) {
+  if (path.empty()) return false;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream ls(line);
+    std::string k;
+    int m = -1, pp = 0;
+    if (!(ls >> k >> m >> pp) || k != key) continue;
+    *mode = m;
+    *parts = pp;
+    return true;
+  }
+  return false;
+}
+
+void tune_store(const std::string& path, const std::string& key, int mode, int parts) {
+  if (path.empty()) return;
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    // create the directory chain (best effort)
+    std::string dir = path.substr(0, slash);
+    for (size_t i = 1; i <= dir.size(); ++i)
+      if (i == dir.size() || dir[i] == '/') (void)::mkdir(dir.substr(0, i).c_str(), 0755);
+  }
+  if (FILE* f = std::fopen(path.c_str(), "a")) {
+    std::fprintf(f, "%s %d %d\n", key.c_str(), mode, parts);
+    std::fclose(f);
+  }
+}
+}  // namespace
+
+template <typename T>
+std::string Spreader<T>::tune_key(int64_t npts) const {
+  std::ostringstream k;
+  k << "spread3:" << (sizeof(T) == 4 ? "f32" : "f64") << ":nf" << geom.nf[0] << "x" << geom.nf[1] << "x" << geom.nf[2]
+    << ":tile" << geom.tile[0] << "x" << geom.tile[1] << "x" << geom.tile[2] << ":w" << es.w << ":batch" << geom.batch
+    << ":n" << npts;
+  return k.str();
+}
+
 // Time every spreader configuration once on the caller's sorted points (a
-// warm-up launch, then one launch between HIP events) and keep the fastest.
+// warm-up launch, then one launch between HIP events) and keep the fastest;
+// a cached pick for the same plan shape is taken without timing.
 // Runs on the first spread of a plan whose grid admits both 3D spreaders.
 template <typename T>
 void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
+  const std::string key = tune_key(p.n);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    int m = -1, pp = 0;
+    if (tune_lookup(user_tune_path(), key, &m, &pp) || tune_lookup(shipped_tune_path(), key, &m, &pp)) {
+      if ((m == REG && geom.tile[1] == 16) || ((m == HALO || m == HALO4) && pp >= 1 && pp <= max_halo_parts)) {
+        mode = m;
+        halo_parts = pp;
+        tuned = true;
+        tune_source = 1;
+        return;
+      }
+    }
+  }
+  tune_source = 2;
+  autotune_timed(p, grid, s);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_store(user_tune_path(), key, mode, halo_parts);
+}
+
+template <typename T>
+void Spreader<T>::autotune_timed(const SortedPoints<T>& p, T* grid, hipStream_t s) const {
   struct Cand {
     int mode, parts;
   };

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -114,8 +115,15 @@ class Spreader {
   // complex strengths str[n] in original point order (times the record
   // weight) onto a complex grid[ncells], fully overwritten.
   void spread_complex(const SortedPoints<T>& p, const Cplx<T>* str, Cplx<T>* grid, hipStream_t s) const;
-  // time the 3D spreader configurations on these points, keep the fastest
+  // 3D spreader configuration for these points: a cached pick for the plan
+  // shape (user cache, then the shipped table), else the fastest of the
+  // candidates timed here (then stored in the user cache)
   void autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
+  void autotune_timed(const SortedPoints<T>& p, T* grid, hipStream_t s) const;
+  std::string tune_key(int64_t npts) const;
+  // where the 3D spreader choice came from: 0 none (single candidate),
+  // 1 tuning cache or table, 2 timed here, 3 pinned by FRESCO_SPREAD3
+  mutable int tune_source = 0;
   // complex spreading of a point set that stays fixed (the k points of
   // UwNU / NUwNU): point-range slices per tile chosen from its bin counts
   // (one host read, at set-up)

@@ -319,6 +319,7 @@ class NUwU final : public Potential {
     *beta = es_.beta;
   }
 
+  int spread_source() const override { return spreader_.halo_ok ? spreader_.tune_source : 0; }
   void spread_config(int* mode, int* parts) const override {
     *mode = spreader_.mode;
     *parts = spreader_.mode == Spreader<T>::REG ? spreader_.parts : spreader_.halo_parts;

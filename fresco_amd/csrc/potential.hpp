@@ -68,6 +68,9 @@ class Potential {
   }
   // spreader configuration in use: mode 0 register tiles / owner tiles,
   // 1 halo tiles; parts = point-range slices per bin (-1 when not applicable)
+  // where spread_config's choice came from: 0 fixed, 1 tuning cache or
+  // table, 2 timed on the first evaluation, 3 pinned by the environment
+  virtual int spread_source() const { return 0; }
   virtual void spread_config(int* mode, int* parts) const {
     *mode = -1;
     *parts = -1;

@@ -242,6 +242,14 @@ int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, v
  */
 int fresco_pot_spread_config(fresco_pot* pot, int* mode, int* parts);
 
+/*
+ * Where that spreader configuration came from: 0 fixed (one candidate), 1 the
+ * autotuning cache or the tuning table shipped with the library (same plan
+ * shape), 2 timed on this plan's first evaluation (then stored in the
+ * cache), 3 pinned by FRESCO_SPREAD3.  No reference counterpart.
+ */
+int fresco_pot_spread_source(fresco_pot* pot, int* source);
+
 /* Number of degrees of freedom the potential expects (n*d for points, prod(N) for grids). */
 int fresco_pot_ndof(fresco_pot* pot, int64_t* ndof);
 
