@@ -2574,6 +2574,229 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
 }
 
 // ---------------------------------------------------------------------------
+// 3D interpolation for kernels 9 to 16 wide (the FP64 grids of eps <= 1e-8),
+// eight lanes per point: lane ja of an 8-lane group takes the x offsets ja
+// and ja + 8 (the second weighted 0 where ja + 8 >= W), sums their W x W
+// cells with the y and z weights shared by ds_swizzle, and the group reduces
+// with three DPP steps, as k_interp3_g8.  The bin's region (tile + W - 1
+// cells per dimension) is staged in LDS by the 256-thread workgroup in
+// batches of row loads; W is a template parameter so that the weight arrays
+// stay in registers.
+// ---------------------------------------------------------------------------
+
+template <typename T, bool DERIV, int W>
+__global__ void __launch_bounds__(G8_THREADS)
+    k_interp3_g16(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
+                  const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
+                  InterpDev<T> a, double* __restrict__ out) {
+  static_assert(W > 8 && W <= 16, "eight lanes cover two x offsets each");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* reg = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.y) / gridDim.y);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.y + 1)) / gridDim.y);
+  if (beg == end) return;
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  const int rx = a.rx, ry = a.ry, srx = a.srx;
+  const int zs = ry * srx;
+  {
+    // thread (row slot, x): rows of up to 32 cells, 8 rows per pass, every
+    // load of a batch issued before its LDS stores
+    constexpr int BATCH = 8;
+    const int lx = tid & 31;
+    int gx = t0[0] + lx;
+    if (gx >= a.nf[0]) gx -= a.nf[0];
+    if (gx >= a.nf[0]) gx -= a.nf[0];
+    const int nrows = ry * a.rz;
+    for (int r0 = tid >> 5; r0 < nrows; r0 += BATCH * (G8_THREADS / 32)) {
+      T v[BATCH];
+#pragma unroll
+      for (int q = 0; q < BATCH; ++q) {
+        const int row = r0 + q * (G8_THREADS / 32);
+        v[q] = T(0);
+        if (row < nrows && lx < rx) {
+          int gy = t0[1] + row % ry;
+          if (gy >= a.nf[1]) gy -= a.nf[1];
+          if (gy >= a.nf[1]) gy -= a.nf[1];
+          int gz = t0[2] + row / ry;
+          if (gz >= a.nf[2]) gz -= a.nf[2];
+          if (gz >= a.nf[2]) gz -= a.nf[2];
+          v[q] = grid[((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < BATCH; ++q) {
+        const int row = r0 + q * (G8_THREADS / 32);
+        if (row < nrows && lx < rx) reg[row * srx + lx] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+  const int ja = tid & 7;
+  const int grp = tid >> 3;
+  const bool two = ja + 8 < W;
+  const int npts = end - beg;
+  const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
+  int kn = beg + grp;
+  uint32_t sc_n = scell[kn < end ? kn : beg];
+  typename Vec4<T>::type r_n = rec[kn < end ? kn : beg];
+  uint32_t j_n = idx[kn < end ? kn : beg];
+  for (int it = 0; it < iters; ++it) {
+    const int k = kn;
+    const bool valid = k < end;
+    const uint32_t sc = sc_n;
+    const typename Vec4<T>::type r = r_n;
+    const uint32_t jout = j_n;
+    kn = k + G8_THREADS / 8;
+    if (it + 1 < iters) {
+      const int kq = kn < end ? kn : beg;
+      sc_n = scell[kq];
+      r_n = rec[kq];
+      j_n = idx[kq];
+    }
+    int l0[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+      if (u < 0) u += a.nf[q];
+      l0[q] = u;
+    }
+    // kernel values of indices ja (set 0) and ja + 8 (set 1)
+    T wx[2], dwx[2], wyl[2], dwyl[2], wzl[2], dwzl[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = ja + 8 * h;
+      wx[h] = dwx[h] = wyl[h] = dwyl[h] = wzl[h] = dwzl[h] = T(0);
+      if (j < W) {
+        if (DERIV) {
+          es_phi_dphi<T>((T(j) - r.x) * a.k.two_over_w, a.k, wx[h], dwx[h]);
+          es_phi_dphi<T>((T(j) - r.y) * a.k.two_over_w, a.k, wyl[h], dwyl[h]);
+          es_phi_dphi<T>((T(j) - r.z) * a.k.two_over_w, a.k, wzl[h], dwzl[h]);
+        } else {
+          wx[h] = es_phi<T>((T(j) - r.x) * a.k.two_over_w, a.k);
+          wyl[h] = es_phi<T>((T(j) - r.y) * a.k.two_over_w, a.k);
+          wzl[h] = es_phi<T>((T(j) - r.z) * a.k.two_over_w, a.k);
+        }
+      }
+    }
+    T wy[16], wz[16], dwy[16], dwz[16];
+    group8_bcast<T>(wyl[0], wy);
+    group8_bcast<T>(wyl[1], wy + 8);
+    group8_bcast<T>(wzl[0], wz);
+    group8_bcast<T>(wzl[1], wz + 8);
+    if (DERIV) {
+      group8_bcast<T>(dwyl[0], dwy);
+      group8_bcast<T>(dwyl[1], dwy + 8);
+      group8_bcast<T>(dwzl[0], dwz);
+      group8_bcast<T>(dwzl[1], dwz + 8);
+    }
+    const T* col0 = reg + (l0[2] * ry + l0[1]) * srx + l0[0] + ja;
+    // the second column of lanes without one reads a staged cell (weight 0)
+    const T* col1 = two ? col0 + 8 : col0;
+    T sA[2] = {T(0), T(0)}, sB[2] = {T(0), T(0)}, sC[2] = {T(0), T(0)};
+#pragma unroll
+    for (int jc = 0; jc < W; ++jc) {
+      T pA[2] = {T(0), T(0)}, pB[2] = {T(0), T(0)};
+#pragma unroll
+      for (int jb = 0; jb < W; ++jb) {
+        const T g0 = col0[jc * zs + jb * srx];
+        const T g1 = col1[jc * zs + jb * srx];
+        pA[0] += g0 * wy[jb];
+        pA[1] += g1 * wy[jb];
+        if (DERIV) {
+          pB[0] += g0 * dwy[jb];
+          pB[1] += g1 * dwy[jb];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        sA[h] += pA[h] * wz[jc];
+        if (DERIV) {
+          sB[h] += pB[h] * wz[jc];
+          sC[h] += pA[h] * dwz[jc];
+        }
+      }
+    }
+    T vx, vy = T(0), vz = T(0);
+    if (DERIV) {
+      vx = sum8_last(dwx[0] * sA[0] + dwx[1] * sA[1]);
+      vy = sum8_last(wx[0] * sB[0] + wx[1] * sB[1]);
+      vz = sum8_last(wx[0] * sC[0] + wx[1] * sC[1]);
+    } else {
+      vx = sum8_last(wx[0] * sA[0] + wx[1] * sA[1]);
+    }
+    if (ja == 7 && valid) {
+      const int64_t j = jout;
+      const double cw = (double)r.w;
+      if (DERIV) {
+        out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
+        out[j * a.ostride + 1] = cw * a.dscale[1] * (double)vy;
+        out[j * a.ostride + 2] = cw * a.dscale[2] * (double)vz;
+      } else {
+        out[j * a.ostride + a.ocomp] = cw * (double)vx;
+      }
+    }
+  }
+}
+
+// true if the region of a kernel w wide fits LDS (then launched)
+template <typename T, bool DERIV>
+static bool launch_interp3_g16(const SortedPoints<T>& p, const T* grid, const double* dscale, double* out,
+                               int ostride, int ocomp, hipStream_t s) {
+  InterpDev<T> a;
+  const GridGeom& g = p.geom;
+  const int w = p.es.w;
+  if (w <= 8 || w > 16) return false;
+  for (int q = 0; q < 3; ++q) {
+    a.nf[q] = g.nf[q];
+    a.tile[q] = g.tile[q];
+    a.ntile[q] = g.ntile[q];
+    a.dscale[q] = dscale ? dscale[q] : 0.0;
+  }
+  a.cbits = g.cell_bits;
+  a.w = w;
+  a.rx = g.tile[0] + w - 1;
+  a.ry = g.tile[1] + w - 1;
+  a.rz = g.tile[2] + w - 1;
+  // lanes with a second column read x offsets up to W - 1 < rx
+  a.srx = a.rx;
+  a.ostride = ostride;
+  a.ocomp = ocomp;
+  a.k.beta = (T)p.es.beta;
+  a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
+  a.k.two_over_w = (T)(2.0 / w);
+  const size_t lds = sizeof(T) * (size_t)a.srx * a.ry * a.rz;
+  if (lds > 160 * 1024 || a.rx > 32 || a.nf[0] < a.rx || a.nf[1] < a.ry || a.nf[2] < a.rz) return false;
+  const int chunks = std::max(1, std::min(8, (4096 + g.nbins - 1) / g.nbins));
+  const dim3 grd(g.nbins, chunks);
+#define FRESCO_G16(W_)                                                                                    \
+  case W_:                                                                                                \
+    hipLaunchKernelGGL((k_interp3_g16<T, DERIV, W_>), grd, dim3(G8_THREADS), lds, s, p.scell.get(),      \
+                       p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);                        \
+    break;
+  switch (w) {
+    FRESCO_G16(9)
+    FRESCO_G16(10)
+    FRESCO_G16(11)
+    FRESCO_G16(12)
+    FRESCO_G16(13)
+    FRESCO_G16(14)
+    FRESCO_G16(15)
+    FRESCO_G16(16)
+  }
+#undef FRESCO_G16
+  FRESCO_HIP_CHECK(hipGetLastError());
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // 2D interpolation, eight lanes per point (w <= 8): the bin's (tile + 7)^2
 // grid region is staged in LDS; lane ja of an 8-lane group sums the 8 cells
 // of its x offset with the y weights (and their derivatives) shared by
@@ -2761,7 +2984,9 @@ void interpolate(const SortedPoints<T>& p, const T* grid, bool deriv, const doub
       deriv ? launch_interp3_lane<T, 8, true>(p, grid, dscale, out, ostride, ocomp, s)
             : launch_interp3_lane<T, 8, false>(p, grid, dscale, out, ostride, ocomp, s);
     } else if (!deriv) {
-      launch_interp3_lane<T, 16, false>(p, grid, dscale, out, ostride, ocomp, s);
+      if (!launch_interp3_g16<T, false>(p, grid, dscale, out, ostride, ocomp, s))
+        launch_interp3_lane<T, 16, false>(p, grid, dscale, out, ostride, ocomp, s);
+    } else if (launch_interp3_g16<T, true>(p, grid, dscale, out, ostride, ocomp, s)) {
     } else if (sizeof(T) == 4) {
       launch_interp3_lane<T, 16, true>(p, grid, dscale, out, ostride, ocomp, s);
     } else {
