@@ -149,7 +149,134 @@ __global__ void __launch_bounds__(BD_T)
   }
 }
 
-template <typename T, bool IN_REAL, bool OUT_REAL, bool LANE_K, int NPT>
+// The same contraction on the matrix cores (FP32): v_mfma_f32_16x16x4f32
+// takes A[m][q] (m = i, q = j) in lane m + 16 q and B[q][n] (n = k) in lane
+// n + 16 q, and returns D[m][n] in register m % 4 of lane n + 16 (m / 4).  A
+// complex product is four real ones (two for a real input or a real output):
+//   Re += Ar Wr - Ai Wi,  Im += Ar Wi + Ai Wr.
+// A workgroup (4 wavefronts) stages a [32 x J] slab of the input and the
+// [32 x J] twiddle rows in LDS (rows of MF_JP(J) entries, J padded with zeros
+// to a multiple of 4) and computes a 32 x 32 block of outputs, a 16 x 16 tile
+// per wavefront.
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int MF_T = 256;
+constexpr int MF_TILE = 32;
+
+// LDS row length in entries: J padded to a multiple of 4 and to 4 mod 32, so
+// that the 16 rows x 4 entries of one operand read spread over the banks
+__host__ __device__ constexpr int mf_jp(int J) { return ((J + 31) / 32) * 32 + 4; }
+
+// TR: the matrix roles swapped (M = k, N = i: the output tile's lanes run
+// along i, for outputs whose unit stride is i)
+template <bool IN_REAL, bool OUT_REAL, bool TR>
+__global__ void __launch_bounds__(MF_T)
+    k_band_mfma(const void* __restrict__ in_, int64_t so, int64_t si, int64_t sj, int I, int J,
+                const Cplx<float>* __restrict__ W, int K, void* __restrict__ out_, int64_t to, int64_t ti,
+                int64_t tk, Cplx<float>* __restrict__ zero_out) {
+  using IT = typename std::conditional<IN_REAL, float, Cplx<float>>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int JP = mf_jp(J);
+  IT* slab = reinterpret_cast<IT*>(smem);  // [32][JP]
+  Cplx<float>* ws = reinterpret_cast<Cplx<float>*>(smem + (((size_t)MF_TILE * JP * sizeof(IT) + 15) & ~(size_t)15));
+  const IT* in = static_cast<const IT*>(in_);
+  const int o = blockIdx.x;
+  const int i0 = blockIdx.y * MF_TILE, k0 = blockIdx.z * MF_TILE;
+  const int tid = threadIdx.x;
+  const int64_t ob = (int64_t)o * so;
+  const int J4 = (J + 3) & ~3;
+  // staging: the unit-stride index runs over consecutive threads; entries
+  // j in [J, J4) and rows beyond I / K are zero; every load of a thread is
+  // issued before its LDS stores (one memory latency per workgroup)
+  constexpr int PER = MF_TILE * BD_JMAX / MF_T;
+  {
+    IT v[PER];
+    Cplx<float> wv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * MF_T;
+      v[u] = IT{};
+      wv[u] = Cplx<float>{0.f, 0.f};
+      if (e < MF_TILE * J4) {
+        int r, j;
+        if (sj == 1) {
+          r = e / J4;
+          j = e - r * J4;
+        } else {
+          j = e / MF_TILE;
+          r = e - j * MF_TILE;
+        }
+        if (i0 + r < I && j < J) v[u] = in[ob + (int64_t)(i0 + r) * si + (int64_t)j * sj];
+        const int rw = e / J4, jw = e - rw * J4;
+        if (k0 + rw < K && jw < J) wv[u] = W[(int64_t)(k0 + rw) * J + jw];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * MF_T;
+      if (e < MF_TILE * J4) {
+        int r, j;
+        if (sj == 1) {
+          r = e / J4;
+          j = e - r * J4;
+        } else {
+          j = e / MF_TILE;
+          r = e - j * MF_TILE;
+        }
+        slab[r * JP + j] = v[u];
+        const int rw = e / J4, jw = e - rw * J4;
+        ws[rw * JP + jw] = wv[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  const int wi = wv & 1, wk = wv >> 1;
+  const int m = lane & 15, q = lane >> 4;
+  // both operands are read as [16 rows][4 j] blocks: row m of the input slab
+  // and row m of the twiddles (A and B of the product, in either order)
+  const IT* arow = slab + (wi * 16 + m) * JP + q;
+  const Cplx<float>* brow = ws + (wk * 16 + m) * JP + q;
+  mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+  auto mf = [](float a, float b, mf_f4 c) {
+    return TR ? __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0)
+              : __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  };
+  for (int j = 0; j < J4; j += 4) {
+    const IT av = arow[j];
+    const Cplx<float> bv = brow[j];
+    if constexpr (IN_REAL) {
+      cr = mf(av, bv.re, cr);
+      ci = mf(av, bv.im, ci);
+    } else {
+      cr = mf(av.re, bv.re, cr);
+      cr = mf(av.im, -bv.im, cr);
+      if constexpr (!OUT_REAL) {
+        ci = mf(av.re, bv.im, ci);
+        ci = mf(av.im, bv.re, ci);
+      }
+    }
+  }
+  // D[m'][n'] in register m' % 4 of lane n' + 16 (m' / 4): (m', n') = (i, k),
+  // or (k, i) for TR
+  const int kr = TR ? 4 * q : m;
+  const int ir = TR ? m : 4 * q;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + wi * 16 + ir + (TR ? 0 : r);
+    const int k = k0 + wk * 16 + kr + (TR ? r : 0);
+    if (i < I && k < K) {
+      const int64_t at = (int64_t)o * to + (int64_t)i * ti + (int64_t)k * tk;
+      if constexpr (OUT_REAL)
+        static_cast<float*>(out_)[at] = cr[r];
+      else
+        static_cast<Cplx<float>*>(out_)[at] = Cplx<float>{cr[r], ci[r]};
+      if (zero_out) zero_out[at] = Cplx<float>{0.f, 0.f};
+    }
+  }
+}
+
+template <typename T, bool IN_REAL, bool OUT_REAL, bool LANE_K, int NPT, bool TR = false>
 void launch_pass(const void* in, int64_t so, int64_t si, int64_t sj, int O, int I, int J, const Cplx<T>* W, int K,
                  void* out, int64_t to, int64_t ti, int64_t tk, hipStream_t s, Cplx<T>* zero_out = nullptr) {
   using IT = typename std::conditional<IN_REAL, T, Cplx<T>>::type;
@@ -158,10 +285,210 @@ void launch_pass(const void* in, int64_t so, int64_t si, int64_t sj, int O, int 
   const size_t lds = (((size_t)J * ITILE * sizeof(IT) + 15) & ~(size_t)15) + (size_t)KTILE * (J | 1) * sizeof(Cplx<T>);
   if (lds > 160 * 1024 || J > BD_JMAX)
     throw InvalidArgument("band transform pass: contracted axis over " + std::to_string(BD_JMAX) + " or LDS exceeded");
+  if constexpr (std::is_same<T, float>::value) {
+    if (J <= BD_JMAX) {
+      const size_t mlds = (((size_t)MF_TILE * mf_jp(J) * sizeof(IT) + 15) & ~(size_t)15) +
+                          (size_t)MF_TILE * mf_jp(J) * sizeof(Cplx<float>);
+      const dim3 mgrd(O, (I + MF_TILE - 1) / MF_TILE, (K + MF_TILE - 1) / MF_TILE);
+      hipLaunchKernelGGL((k_band_mfma<IN_REAL, OUT_REAL, TR>), mgrd, dim3(MF_T), mlds, s, in, so, si, sj, I, J, W, K, out,
+                         to, ti, tk, zero_out);
+      FRESCO_HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const dim3 grd(O, (I + ITILE - 1) / ITILE, (K + KTILE - 1) / KTILE);
   hipLaunchKernelGGL((k_band_pass<T, IN_REAL, OUT_REAL, LANE_K, NPT>), grd, dim3(BD_T), lds, s, in, so, si, sj, I, J,
                      W, K, out, to, ti, tk, zero_out);
   FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Fused y / z passes of one kx plane on the matrix cores (FP32), one
+// workgroup of 4 wavefronts per (kx, 16-wide tile of the other output index):
+//   forward  T[z][ky] = sum_y U1[kx][z][y] Wy[ky][y]      (ky tile in LDS)
+//            S[kz][ky][kx] = sum_z Wz[kz][z] T[z][ky]
+//   inverse  Z1[kz][y] = sum_ky X[kz][ky][kx] Wyi[y][ky]  (y tile in LDS)
+//            Y2[kx][z][y] = sum_kz Wzi[z][kz] Z1[kz][y]
+// The contraction over the short index runs first (the inverse contracts
+// ky before kz), the intermediate stays in LDS, and the A operand of every
+// 16 x 16 tile (one 8-byte value per lane and 4-deep step) is loaded for all
+// its steps before the first product.
+// ---------------------------------------------------------------------------
+
+constexpr int FZ_MAXS = 32;  // 4-deep steps of a contraction (indices up to 128)
+
+// one 16 x 16 complex tile: cr/ci += A B with A[m][q] = av[s] (lane m + 16 q,
+// step s) and B[q][n] = bp[(4 s + q) * bstride + n-th column offset]
+__device__ __forceinline__ void mf_tile(const Cplx<float>* av, int nsteps, const Cplx<float>* bcol, int bstep,
+                                        mf_f4& cr, mf_f4& ci) {
+#pragma unroll
+  for (int st = 0; st < FZ_MAXS; ++st) {
+    if (st < nsteps) {
+      const Cplx<float> a = av[st];
+      const Cplx<float> b = bcol[st * bstep];
+      cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.re, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.im, -b.im, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.im, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.im, b.re, ci, 0, 0, 0);
+    }
+  }
+}
+
+// U1 [nb][nz][ny], Wy [nk][ny], Wz [nk][nz] -> S [nk][nk][nb] (kz, ky, kx);
+// zero_out (optional, same layout as S) cleared where S is written
+__global__ void __launch_bounds__(256)
+    k_band_fwd_yz(const Cplx<float>* __restrict__ U1, int ny, int nz, int nb, int nk,
+                  const Cplx<float>* __restrict__ Wy, const Cplx<float>* __restrict__ Wz, Cplx<float>* __restrict__ S,
+                  Cplx<float>* __restrict__ zero_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int kx = blockIdx.x;
+  const int ky0 = blockIdx.y * 16;
+  const int ny4 = (ny + 3) & ~3, nz4 = (nz + 3) & ~3;
+  const int JPY = mf_jp(ny);
+  Cplx<float>* wyl = reinterpret_cast<Cplx<float>*>(smem);  // [16][JPY]: Wy rows ky0..ky0+15
+  Cplx<float>* T = wyl + 16 * JPY;                           // [nz4 rounded to 16][16]: z x ky
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  {
+    constexpr int PER = 16 * 128 / 256;
+    Cplx<float> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int r = e / ny4, y = e - r * ny4;
+      v[u] = Cplx<float>{0.f, 0.f};
+      if (e < 16 * ny4 && ky0 + r < nk && y < ny) v[u] = Wy[(int64_t)(ky0 + r) * ny + y];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int r = e / ny4, y = e - r * ny4;
+      if (e < 16 * ny4) wyl[r * JPY + y] = v[u];
+    }
+  }
+  __syncthreads();
+  // step A: wavefront wv takes z tiles wv, wv + 4, ...
+  const int nzt = (nz + 15) / 16;
+  const int sy = ny4 / 4;
+  const Cplx<float>* u1 = U1 + (int64_t)kx * nz * ny;
+  for (int zt = wv; zt < nzt; zt += 4) {
+    const int z = zt * 16 + m;
+    Cplx<float> av[FZ_MAXS];
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int y = 4 * st + q;
+      av[st] = Cplx<float>{0.f, 0.f};
+      if (st < sy && z < nz && y < ny) av[st] = u1[(int64_t)z * ny + y];
+    }
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    // B[q][n] = Wy[ky0 + n][4 st + q]: lane n + 16 q reads row n
+    mf_tile(av, sy, wyl + m * JPY + q, 4, cr, ci);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) T[(zt * 16 + 4 * q + r) * 16 + m] = Cplx<float>{cr[r], ci[r]};
+  }
+  __syncthreads();
+  // step B: wavefront wv takes kz tiles wv, wv + 4, ...
+  const int nkt = (nk + 15) / 16;
+  const int sz = nz4 / 4;
+  for (int kt = wv; kt < nkt; kt += 4) {
+    const int kz = kt * 16 + m;
+    Cplx<float> av[FZ_MAXS];
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int z = 4 * st + q;
+      av[st] = Cplx<float>{0.f, 0.f};
+      if (st < sz && kz < nk && z < nz) av[st] = Wz[(int64_t)kz * nz + z];
+    }
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    // B[q][n] = T[4 st + q][n]
+    mf_tile(av, sz, T + q * 16 + m, 64, cr, ci);
+    const int ky = ky0 + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kzo = kt * 16 + 4 * q + r;
+      if (kzo < nk && ky < nk) {
+        const int64_t at = ((int64_t)kzo * nk + ky) * nb + kx;
+        S[at] = Cplx<float>{cr[r], ci[r]};
+        if (zero_out) zero_out[at] = Cplx<float>{0.f, 0.f};
+      }
+    }
+  }
+}
+
+// X [nk][nk][nb] (kz, ky, kx), Wyi [ny][nk], Wzi [nz][nk] -> Y2 [nb][nz][ny]
+__global__ void __launch_bounds__(256)
+    k_band_inv_zy(const Cplx<float>* __restrict__ X, int ny, int nz, int nb, int nk,
+                  const Cplx<float>* __restrict__ Wyi, const Cplx<float>* __restrict__ Wzi,
+                  Cplx<float>* __restrict__ Y2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int kx = blockIdx.x;
+  const int y0 = blockIdx.y * 16;
+  const int nk4 = (nk + 3) & ~3;
+  const int JPK = mf_jp(nk);
+  Cplx<float>* wyl = reinterpret_cast<Cplx<float>*>(smem);  // [16][JPK]: Wyi rows y0..y0+15
+  Cplx<float>* Z1 = wyl + 16 * JPK;                          // [nk4 rounded to 16][16]: kz x y
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  {
+    constexpr int PER = 16 * 128 / 256;
+    Cplx<float> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int r = e / nk4, j = e - r * nk4;
+      v[u] = Cplx<float>{0.f, 0.f};
+      if (e < 16 * nk4 && y0 + r < ny && j < nk) v[u] = Wyi[(int64_t)(y0 + r) * nk + j];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int r = e / nk4, j = e - r * nk4;
+      if (e < 16 * nk4) wyl[r * JPK + j] = v[u];
+    }
+  }
+  // Z1 rows beyond nk stay zero for the kz steps that run past them
+  const int nkt = (nk + 15) / 16;
+  for (int e = tid; e < nkt * 16 * 16; e += 256) Z1[e] = Cplx<float>{0.f, 0.f};
+  __syncthreads();
+  const int sk = nk4 / 4;
+  // step A: Z1[kz][y] over ky; wavefront wv takes kz tiles wv, wv + 4, ...
+  for (int kt = wv; kt < nkt; kt += 4) {
+    const int kz = kt * 16 + m;
+    Cplx<float> av[FZ_MAXS];
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int ky = 4 * st + q;
+      av[st] = Cplx<float>{0.f, 0.f};
+      if (st < sk && kz < nk && ky < nk) av[st] = X[((int64_t)kz * nk + ky) * nb + kx];
+    }
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    // B[q][n] = Wyi[y0 + n][4 st + q]
+    mf_tile(av, sk, wyl + m * JPK + q, 4, cr, ci);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Z1[(kt * 16 + 4 * q + r) * 16 + m] = Cplx<float>{cr[r], ci[r]};
+  }
+  __syncthreads();
+  // step B: Y2[z][y] over kz; wavefront wv takes z tiles wv, wv + 4, ...
+  const int nzt = (nz + 15) / 16;
+  for (int zt = wv; zt < nzt; zt += 4) {
+    const int z = zt * 16 + m;
+    Cplx<float> av[FZ_MAXS];
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int kz = 4 * st + q;
+      av[st] = Cplx<float>{0.f, 0.f};
+      if (st < sk && z < nz && kz < nk) av[st] = Wzi[(int64_t)z * nk + kz];
+    }
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    // B[q][n] = Z1[4 st + q][n]
+    mf_tile(av, sk, Z1 + q * 16 + m, 64, cr, ci);
+    const int y = y0 + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int zo = zt * 16 + 4 * q + r;
+      if (zo < nz && y < ny) Y2[((int64_t)kx * nz + zo) * ny + y] = Cplx<float>{cr[r], ci[r]};
+    }
+  }
 }
 
 // W[k][j] = wt(j') exp(sign 2 pi i k' j' / n) with k' = k + koff, j' = j + joff
@@ -206,6 +533,17 @@ void BandDft<T>::init(const int* nf, int band) {
 template <typename T>
 void BandDft<T>::forward(const T* grid, Cplx<T>* S, hipStream_t s, Cplx<T>* zero) {
   const int nx = nf_[0], ny = nf_[1], nz = nf_[2], nk = 2 * B_ + 1, nb = nb_;
+  if constexpr (std::is_same<T, float>::value) {
+    // x: grid[z][y][x] -> U1[kx][z][y] (output lanes along y); then y and z
+    // of every kx plane in one launch
+    launch_pass<T, true, false, true, BD_NPT, true>(grid, (int64_t)ny * nx, nx, 1, nz, ny, nx, wa_.get(), nb,
+                                                    u1_.get(), ny, 1, (int64_t)nz * ny, s);
+    const size_t lds = sizeof(Cplx<float>) * (16 * (size_t)mf_jp(ny) + (size_t)((nz + 15) / 16) * 16 * 16);
+    hipLaunchKernelGGL(k_band_fwd_yz, dim3(nb, (nk + 15) / 16), dim3(256), lds, s, u1_.get(), ny, nz, nb, nk,
+                       wb_.get(), wc_.get(), S, zero);
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return;
+  }
   // x: grid[z][y][x] -> U1[z][y][kx]; lanes over kx (coalesced writes)
   launch_pass<T, true, false, true, BD_NPT>(grid, (int64_t)ny * nx, nx, 1, nz, ny, nx, wa_.get(), nb, u1_.get(),
                                        (int64_t)ny * nb, nb, 1, s);
@@ -220,6 +558,17 @@ void BandDft<T>::forward(const T* grid, Cplx<T>* S, hipStream_t s, Cplx<T>* zero
 template <typename T>
 void BandDft<T>::inverse(const Cplx<T>* X, T* grid, hipStream_t s) {
   const int nx = nf_[0], ny = nf_[1], nz = nf_[2], nk = 2 * B_ + 1, nb = nb_;
+  if constexpr (std::is_same<T, float>::value) {
+    // ky then kz of every kx plane in one launch: X -> Y2[kx][z][y] (in u1_)
+    const size_t lds = sizeof(Cplx<float>) * (16 * (size_t)mf_jp(nk) + (size_t)((nk + 15) / 16) * 16 * 16);
+    hipLaunchKernelGGL(k_band_inv_zy, dim3(nb, (ny + 15) / 16), dim3(256), lds, s, X, ny, nz, nb, nk, wbi_.get(),
+                       wci_.get(), u1_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    // x: Y2[kx][z][y] -> grid[z][y][x] (real part, kx > 0 twice)
+    launch_pass<T, false, true, true, BD_NPT>(u1_.get(), ny, 1, (int64_t)nz * ny, nz, ny, nb, wai_.get(), nx, grid,
+                                              (int64_t)ny * nx, nx, 1, s);
+    return;
+  }
   // z: X[kz][ky][kx] -> Y1[z][ky][kx] (in u2_)
   launch_pass<T, false, false, false, BD_NPT>(X, nb, 1, (int64_t)nk * nb, nk, nb, nk, wci_.get(), nz, u2_.get(), nb, 1,
                                          (int64_t)nk * nb, s);

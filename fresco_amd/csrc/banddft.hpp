@@ -8,6 +8,10 @@
 //   forward  grid[z][y][x] -> U1[z][y][kx] -> U2[z][ky][kx] -> S[kz][ky][kx]
 //   inverse  X[kz][ky][kx] -> Y1[z][ky][kx] -> Y2[z][y][kx] -> grid[z][y][x]
 // with kx in [0, B] (real input: S(-k) = conj S(k)) and ky, kz in [-B, B].
+// FP32 runs on the matrix cores (v_mfma_f32_16x16x4f32) in four launches:
+// the x pass, the y and z passes of every kx plane fused (the intermediate
+// in LDS), and the same in reverse; FP64 keeps three VALU passes per
+// direction.
 // Signs and scaling are those of the FFTs they replace: forward e^{-2 pi i
 // k x / n}, inverse e^{+2 pi i k x / n} unnormalised, the inverse taking the
 // kx = 0 plane as it is (Hermitian by construction) and kx > 0 twice (their
