@@ -3179,31 +3179,7 @@ void fill_zero(T* p, size_t n, hipStream_t s) {
 }
 
 __global__ void k_reduce_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
-  __shared__ double ss[256];
-  __shared__ double cc[256];
-  double sum = 0.0, comp = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double y = part[i] - comp;
-    double t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
-  }
-  ss[threadIdx.x] = sum;
-  cc[threadIdx.x] = comp;
-  __syncthreads();
-  for (int h = 128; h >= 1; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      // two compensated partial sums: add with a TwoSum and keep the error
-      double a = ss[threadIdx.x], b = ss[threadIdx.x + h];
-      double s2 = a + b;
-      double bb = s2 - a;
-      double err = (a - (s2 - bb)) + (b - bb);
-      ss[threadIdx.x] = s2;
-      cc[threadIdx.x] = cc[threadIdx.x] + cc[threadIdx.x + h] - err;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = ss[0] - cc[0];
+  block_sum_partials(part, n, out, nullptr);
 }
 
 void reduce_partials(const double* partials, int n, double* out, hipStream_t s) {

@@ -194,4 +194,61 @@ void fill_zero(T* p, size_t n, hipStream_t s);
 // Deterministic FP64 sum of partials[n] -> *out (device), Kahan-compensated.
 void reduce_partials(const double* partials, int n, double* out, hipStream_t s);
 
+// The same compensated, fixed-order sum by one 256-thread workgroup (every
+// thread calls it): part[n] -> *out, and *out_host (device-visible pinned
+// host memory) when not null.
+__device__ __forceinline__ void block_sum_partials(const double* part, int n, double* out, double* out_host) {
+  __shared__ double ss[256];
+  __shared__ double cc[256];
+  double sum = 0.0, comp = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double y = part[i] - comp;
+    double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+  }
+  ss[threadIdx.x] = sum;
+  cc[threadIdx.x] = comp;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      // two compensated partial sums: add with a TwoSum and keep the error
+      double a = ss[threadIdx.x], b = ss[threadIdx.x + h];
+      double s2 = a + b;
+      double bb = s2 - a;
+      double err = (a - (s2 - bb)) + (b - bb);
+      ss[threadIdx.x] = s2;
+      cc[threadIdx.x] = cc[threadIdx.x] + cc[threadIdx.x + h] - err;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double e = ss[0] - cc[0];
+    out[0] = e;
+    if (out_host) {
+      out_host[0] = e;
+      __threadfence_system();
+    }
+  }
+}
+
+// true in the last of nblocks workgroups to arrive (each after its thread 0
+// wrote its partial); that workgroup then reads every partial.  The ticket
+// (device memory, zero before the first launch) is reset by the last one, so
+// the launch can be repeated or replayed in a graph.
+__device__ __forceinline__ bool last_block_arrived(unsigned* ticket, unsigned nblocks) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = t == nblocks - 1;
+    if (last) {
+      __threadfence();
+      *ticket = 0u;
+    }
+  }
+  __syncthreads();
+  return last;
+}
+
 }  // namespace fresco_hip

@@ -47,7 +47,8 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
     k_nuwu_loss(const Cplx<T>* __restrict__ S, PairDev pd, int npairs, double norm, double gscale,
                 Cplx<T>* __restrict__ X0, Cplx<T>* __restrict__ X1, Cplx<T>* __restrict__ X2, int d,
-                double* __restrict__ partial) {
+                double* __restrict__ partial, unsigned* __restrict__ ticket, double* __restrict__ e_out,
+                double* __restrict__ e_host) {
   __shared__ double red[256];
   const int i = blockIdx.x * 256 + threadIdx.x;
   double e = 0.0;
@@ -116,6 +117,9 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  // the last workgroup sums the partials in their fixed order (the energy of
+  // the evaluation, nuwu.hpp:270-271), with no extra launch
+  if (last_block_arrived(ticket, gridDim.x)) block_sum_partials(partial, gridDim.x, e_out, e_host);
 }
 
 struct RhoDev {
@@ -263,19 +267,18 @@ class NUwU final : public Potential {
     }
     {
       ProfScope p(ST_LOSS, s);
-      launch_loss<0>(s);
-      reduce_partials(partial_.get(), nblk_, e_dev_.get(), s);
+      launch_loss<0>(s, e_dev_.get(), host_.dev());
     }
-    return host_.fetch(e_dev_.get(), s);
+    return host_.wait(s);
   }
 
   double energy_gradient(const double* x, double* grad, hipStream_t s) override {
-    enqueue_eg(x, grad, e_dev_.get(), s);
-    return host_.fetch(e_dev_.get(), s);
+    enqueue_eg(x, grad, e_dev_.get(), host_.dev(), s);
+    return host_.wait(s);
   }
 
   bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
-    enqueue_eg(x, grad, e_dev, s);
+    enqueue_eg(x, grad, e_dev, nullptr, s);
     return true;
   }
   // the spreader's first call may time its candidates (host waits)
@@ -359,7 +362,9 @@ class NUwU final : public Potential {
   }
 
  private:
-  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
+  // e_host: device-visible pinned host memory that also receives the energy
+  // (the host-synchronous entry points), or null (device-only: graphs)
+  void enqueue_eg(const double* x, double* grad, double* e_out, double* e_host, hipStream_t s) {
     last_x_ = x;
     {
       ProfScope p(ST_SORT, s);
@@ -381,10 +386,9 @@ class NUwU final : public Potential {
     {
       ProfScope p(ST_LOSS, s);
       if (plain_)
-        launch_loss<2>(s);
+        launch_loss<2>(s, e_out, e_host);
       else
-        launch_loss<1>(s);
-      reduce_partials(partial_.get(), nblk_, e_out, s);
+        launch_loss<1>(s, e_out, e_host);
     }
     if (plain_) {
       for (int a = 0; a < d_; ++a) {
@@ -473,12 +477,12 @@ class NUwU final : public Potential {
   }
 
   template <int MODE>
-  void launch_loss(hipStream_t s) {
+  void launch_loss(hipStream_t s, double* e_out, double* e_host) {
     Cplx<T>* X0 = X_[0].get();
     Cplx<T>* X1 = nX_ > 1 ? X_[1].get() : nullptr;
     Cplx<T>* X2 = nX_ > 2 ? X_[2].get() : nullptr;
     hipLaunchKernelGGL((k_nuwu_loss<T, MODE>), dim3(nblk_), dim3(256), 0, s, S_.get(), pd_, npairs_,
-                       (double)N_, 4.0 / (double)N_, X0, X1, X2, d_, partial_.get());
+                       (double)N_, 4.0 / (double)N_, X0, X1, X2, d_, partial_.get(), ticket_.get(), e_out, e_host);
     FRESCO_HIP_CHECK(hipGetLastError());
   }
 
@@ -552,6 +556,10 @@ class NUwU final : public Potential {
                   p_vm_.get(), p_sk_.get(), p_sm_.get(), p_ip_.get()};
     nblk_ = std::max(1, (npairs_ + 255) / 256);
     partial_.allocate(nblk_);
+    if (!ticket_.get()) {
+      ticket_.allocate(1);
+      FRESCO_HIP_CHECK(hipMemset(ticket_.get(), 0, sizeof(unsigned)));
+    }
     e_dev_.allocate(1);
   }
 
@@ -582,6 +590,7 @@ class NUwU final : public Potential {
   PairDev pd_{};
   int npairs_ = 0, nblk_ = 1;
   DeviceBuffer<double> partial_, e_dev_;
+  DeviceBuffer<unsigned> ticket_;  // last-workgroup ticket of the loss kernel
   DeviceBuffer<double> Sk_full_, V_full_, rho_tmp_;  // targets on the full mode grid
   DeviceBuffer<unsigned char> active_;               // mode belongs to an active pair
   DeviceBuffer<unsigned int> count_;

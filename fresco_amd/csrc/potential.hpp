@@ -100,9 +100,21 @@ class HostScalar {
     if (prof_enabled()) prof_collect();
     return *p_;
   }
+  // the scalar as kernels address it (a kernel writes it directly: no copy)
+  double* dev() {
+    if (!d_) FRESCO_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_), p_, 0));
+    return d_;
+  }
+  // after a kernel enqueued on s wrote it through dev()
+  double wait(hipStream_t s) {
+    FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+    if (prof_enabled()) prof_collect();
+    return *p_;
+  }
 
  private:
   double* p_ = nullptr;
+  double* d_ = nullptr;
 };
 
 // Per-mode derivatives of phi = sum V D^2, D = (|rho|^2/norm - S0)[/S0 where
