@@ -2361,6 +2361,7 @@ static void launch_interp3_lane(const SortedPoints<T>& p, const T* grid, const d
 // ---------------------------------------------------------------------------
 
 constexpr int G8_THREADS = 256;
+constexpr int G8_SORT_MAX = 1024;  // slice points reordered in LDS
 
 template <typename T>
 __device__ __forceinline__ T sum8_last(T v) {
@@ -2442,22 +2443,69 @@ __global__ void __launch_bounds__(G8_THREADS)
   const int srx = SRX ? SRX : a.srx;
   const int zs = (RY ? RY : a.ry) * srx;
   const int npts = end - beg;
+  // The 8 points one wavefront interpolates together read 8-cell windows of
+  // the region; at unrelated (y, z) rows those windows collide in the LDS
+  // banks.  Points taken in the order of their first (z, y) cell share their
+  // rows, and then every read of the wavefront hits distinct banks or the
+  // same words.  Each point's result does not depend on the order, so the
+  // slice is reordered by a counting sort in LDS (slices up to G8_SORT_MAX
+  // points; longer ones keep the bin order).
+  __shared__ uint16_t perm[G8_SORT_MAX];
+  __shared__ int kcnt[128];
+  const bool sorted = npts <= G8_SORT_MAX;
+  if (sorted) {
+    if (tid < 128) kcnt[tid] = 0;
+    __syncthreads();
+    int key[G8_SORT_MAX / G8_THREADS];
+#pragma unroll
+    for (int u = 0; u < G8_SORT_MAX / G8_THREADS; ++u) {
+      const int e = tid + u * G8_THREADS;
+      key[u] = -1;
+      if (e < npts) {
+        const uint32_t sc = scell[beg + e];
+        const int uy = (int)((sc >> a.cbits) & cmask) - t0[1];
+        const int uz = (int)((sc >> (2 * a.cbits)) & cmask) - t0[2];
+        key[u] = ((uz & 7) << 4) | (uy & 15);
+        atomicAdd(&kcnt[key[u]], 1);
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      // exclusive prefix over the 128 counts, two per lane
+      const int c0 = kcnt[2 * tid], c1 = kcnt[2 * tid + 1];
+      int inc = c0 + c1;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off, 64);
+        if (tid >= off) inc += u;
+      }
+      const int ex = inc - (c0 + c1);
+      kcnt[2 * tid] = ex;
+      kcnt[2 * tid + 1] = ex + c0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < G8_SORT_MAX / G8_THREADS; ++u)
+      if (key[u] >= 0) perm[atomicAdd(&kcnt[key[u]], 1)] = (uint16_t)(tid + u * G8_THREADS);
+    __syncthreads();
+  }
+  auto point_at = [&](int pos) { return pos < npts ? beg + (sorted ? (int)perm[pos] : pos) : beg; };
   // every group runs the same number of iterations (DPP needs all lanes)
   const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
   // the next iteration's point is loaded while this one is interpolated
-  int kn = beg + grp;
-  uint32_t sc_n = scell[kn < end ? kn : beg];
-  typename Vec4<T>::type r_n = rec[kn < end ? kn : beg];
-  uint32_t j_n = idx[kn < end ? kn : beg];
+  int pn = grp;
+  int kq0 = point_at(pn);
+  uint32_t sc_n = scell[kq0];
+  typename Vec4<T>::type r_n = rec[kq0];
+  uint32_t j_n = idx[kq0];
   for (int it = 0; it < iters; ++it) {
-    const int k = kn;
-    const bool valid = k < end;
+    const bool valid = pn < npts;
     const uint32_t sc = sc_n;
     const typename Vec4<T>::type r = r_n;
     const uint32_t jout = j_n;
-    kn = k + G8_THREADS / 8;
+    pn += G8_THREADS / 8;
     if (it + 1 < iters) {
-      const int kq = kn < end ? kn : beg;
+      const int kq = point_at(pn);
       sc_n = scell[kq];
       r_n = rec[kq];
       j_n = idx[kq];
