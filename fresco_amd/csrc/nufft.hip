@@ -14,6 +14,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <type_traits>
 
 #include "nufft.hpp"
 #include "prof.hpp"
@@ -1036,12 +1037,18 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
-    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] pins the choice (reproducible
-    // rounding across processes) instead of timing the candidates
+    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] | halom[:<slices>] pins the
+    // choice (reproducible rounding across processes) instead of timing the
+    // candidates; halom (FP32 strengths, 16 x 8 x 8 bins) falls back to halo4 otherwise
     if (const char* env = std::getenv("FRESCO_SPREAD3")) {
       const std::string v(env);
       if (v == "reg") {
         mode = REG;
+      } else if (v.rfind("halom", 0) == 0) {
+        // halom:<parts>: two x windows per slice, so parts is even
+        mode = sizeof(T) == 4 && g.tile[1] == 8 ? HALOM : HALO4;
+        if (v.size() > 6) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 6)));
+        if (mode == HALOM) halo_parts = std::max(2, std::min(max_halo_parts & ~1, (halo_parts + 1) & ~1));
       } else if (v.rfind("halo4", 0) == 0) {
         mode = HALO4;
         if (v.size() > 6) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 6)));
@@ -1141,7 +1148,9 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
     std::lock_guard<std::mutex> lock(g_tune_mu);
     int m = -1, pp = 0;
     if (tune_lookup(user_tune_path(), key, &m, &pp) || tune_lookup(shipped_tune_path(), key, &m, &pp)) {
-      if ((m == REG && geom.tile[1] == 16) || ((m == HALO || m == HALO4) && pp >= 1 && pp <= max_halo_parts)) {
+      const bool halo_mode =
+          m == HALO || m == HALO4 || (m == HALOM && sizeof(T) == 4 && geom.tile[1] == 8 && pp % 2 == 0);
+      if ((m == REG && geom.tile[1] == 16) || (halo_mode && pp >= 1 && pp <= max_halo_parts)) {
         mode = m;
         halo_parts = pp;
         tuned = true;
@@ -1167,6 +1176,9 @@ void Spreader<T>::autotune_timed(const SortedPoints<T>& p, T* grid, hipStream_t 
     if (sl <= max_halo_parts) cands.push_back({HALO, sl});
   for (int sl : {1, 2, 3, 4})
     if (sl <= max_halo_parts) cands.push_back({HALO4, sl});
+  if (sizeof(T) == 4 && geom.tile[1] == 8)
+    for (int sl : {1, 2, 3})
+      if (2 * sl <= max_halo_parts) cands.push_back({HALOM, 2 * sl});
   hipEvent_t e0, e1;
   FRESCO_HIP_CHECK(hipEventCreate(&e0));
   FRESCO_HIP_CHECK(hipEventCreate(&e1));
@@ -1969,6 +1981,236 @@ __global__ void __launch_bounds__(64 * NW)
   for (int e = tid; e < HT_ELEMS; e += NT) out[e] = tile[e];
 }
 
+// The halo spreading of FP32 strengths on the matrix cores (16 x 8 x 8
+// bins), one wavefront per bin slice and x window [8 h, 8 h + 16) (h = 0, 1:
+// the points whose first x cell lies in the bin's left or right 8 columns),
+// the window's 15 planes held in accumulator registers.  Plane z is
+//   D_z[y][j] = sum_p  wy_p[y - uy_p]  *  wz_p[z - uz_p] c_p wx_p[8 h + j - ux_p]
+// with the kernel values zero outside [0, 8): a 16 x 16 product over the
+// points (K, 4 per v_mfma_f32_16x16x4f32), A = the points' y values placed at
+// their first y cell, B = their z-weighted x values placed at their first x
+// cell.  The window's points are ranked by the z of their first cell, each
+// run padded to whole chunks of 8 slots (empty slots weigh 0), so the 4
+// points of a product share it and add to the 8 planes uz .. uz + 7: 8
+// products per 4 points and no read-modify-write of a tile at all.  Each
+// wavefront writes its window's columns of a halo tile (the other columns
+// stay zero), and the merge sums the two windows with the slices.  Ranking
+// is stable (ballot ranks in index order), so the summation order is fixed
+// and the result bitwise reproducible.
+constexpr int HM_WIN = 384;   // points ranked (and staged in LDS) per window of a slice
+constexpr int HM_CH = 8;      // slots per chunk (one key, two products; runs padded to it)
+constexpr int HM_RS = 48;     // floats per slot record: ay[16], bx[16], wz[8]
+constexpr int HM_NK = 8;      // keys: uz
+
+typedef float hm_f4 __attribute__((ext_vector_type(4)));
+
+// f(integral_constant<int, I>) for I = B .. E - 1, in order
+template <int B, int E, typename F>
+__device__ __forceinline__ void hm_static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    hm_static_for<B + 1, E>(f);
+  }
+}
+
+template <int UZ>
+__device__ __forceinline__ void hm_products(hm_f4 (&acc)[HP_Z], float av, float bx, const float4& zlo,
+                                            const float4& zhi) {
+  const float wz[8] = {zlo.x, zlo.y, zlo.z, zlo.w, zhi.x, zhi.y, zhi.z, zhi.w};
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    acc[UZ + m] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx * wz[m], acc[UZ + m], 0, 0, 0);
+}
+
+// grid (bins, 2 * slices): blockIdx.y = 2 slice + h; halo parts = gridDim.y
+__global__ void __launch_bounds__(64)
+    k_spread3_halo_mfma(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
+                        const int* __restrict__ bin_start, SpreadDev<float> a, float* __restrict__ halo) {
+  constexpr int HS_Y = HaloShape<8>::HS_Y;
+  constexpr int ELEMS = HaloShape<8>::ELEMS;
+  static_assert(HS_Y == 16, "the 16 rows of a product are the tile's y rows");
+  constexpr int MAXSLOT = HM_WIN + (HM_CH - 1) * HM_NK;
+  constexpr int WPL = HM_WIN / 64;  // window points per lane
+  __shared__ __align__(16) float recs[2][HM_CH * HM_RS];  // kernel values of two chunks
+  __shared__ __align__(16) float4 wrec[HM_WIN];  // the window's records and first cells
+  __shared__ uint32_t wsc[HM_WIN];
+  __shared__ uint16_t perm[MAXSLOT];
+  __shared__ int kend[HM_NK];
+  __shared__ int run[HM_NK];
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  const int h = blockIdx.y & 1;
+  const int nsl = gridDim.y >> 1, sl = blockIdx.y >> 1;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * sl) / nsl);
+  const int end = beg0 + (int)(((int64_t)len * (sl + 1)) / nsl);
+  float* out = halo + ((int64_t)b * gridDim.y + blockIdx.y) * ELEMS;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  // first cell relative to the bin (u < 0 only across the periodic wrap)
+  auto rel = [&](uint32_t sc, int q) {
+    int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+    if (u < 0) u += a.nf[q];
+    return u;
+  };
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int kj = lane & 7;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  hm_f4 acc[HP_Z];
+#pragma unroll
+  for (int z = 0; z < HP_Z; ++z) acc[z] = hm_f4{0.f, 0.f, 0.f, 0.f};
+  for (int ws = beg; ws < end; ws += HM_WIN) {
+    const int wn = min(HM_WIN, end - ws);
+    // the window's records and first cells, loaded once (coalesced, all
+    // loads in flight together) and staged in LDS for the chunks
+    int wkey[WPL];
+    {
+      float4 wr[WPL];
+#pragma unroll
+      for (int u = 0; u < WPL; ++u) {
+        const int e = u * 64 + lane;
+        wkey[u] = e < wn ? (int)scell[ws + e] : 0;
+        wr[u] = e < wn ? rec[ws + e] : float4{0.f, 0.f, 0.f, 0.f};
+      }
+      __syncthreads();  // the previous window's chunks are done with wrec / wsc
+#pragma unroll
+      for (int u = 0; u < WPL; ++u) {
+        wrec[u * 64 + lane] = wr[u];
+        wsc[u * 64 + lane] = (uint32_t)wkey[u];
+      }
+    }
+    if (lane < HM_NK) run[lane] = 0;
+    // key: z of the first cell, for the points of this x window; -1 others
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      const uint32_t sc = (uint32_t)wkey[u];
+      const bool mine = u * 64 + lane < wn && ((rel(sc, 0) >> 3) & 1) == h;
+      wkey[u] = mine ? (rel(sc, 2) & 7) : -1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WPL; ++u)
+      if (wkey[u] >= 0) atomicAdd(&run[wkey[u]], 1);
+    __syncthreads();
+    // runs padded to whole chunks: exclusive prefix over the keys
+    const int cnt = lane < HM_NK ? run[lane] : 0;
+    const int pad = (cnt + HM_CH - 1) / HM_CH * HM_CH;
+    int inc = pad;
+#pragma unroll
+    for (int off = 1; off < HM_NK; off <<= 1) {
+      const int u = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += u;
+    }
+    const int nslot = __builtin_amdgcn_readfirstlane(__shfl(inc, HM_NK - 1, 64));
+    const int st = inc - pad;
+    __syncthreads();
+    if (lane < HM_NK) {
+      run[lane] = st;
+      kend[lane] = st + pad;
+    }
+    for (int q = lane; q < nslot; q += 64) perm[q] = 0xFFFF;
+    __syncthreads();
+    // stable placement, 64 points a round: rank among equal keys by ballots
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      if (u * 64 >= wn) break;
+      const int key = wkey[u];
+      const bool valid = key >= 0;
+      uint64_t same = __ballot(valid);
+#pragma unroll
+      for (int bt = 0; bt < 3; ++bt) {
+        const bool bit = (key >> bt) & 1;
+        const uint64_t mk = __ballot(bit);
+        same &= bit ? mk : ~mk;
+      }
+      const int rank = __popcll(same & lt);
+      if (valid) perm[run[key] + rank] = (uint16_t)(u * 64 + lane);
+      __syncthreads();
+      if (valid && rank == 0) run[key] += __popcll(same);
+      __syncthreads();
+    }
+    // chunks of 8 slots, each of one key: the products of chunk c (plane
+    // offset known at compile time) run while the kernel values of chunk c + 1
+    // are evaluated into the other record buffer and the points of chunk
+    // c + 2 are read.  One wavefront: its LDS accesses complete in program
+    // order, so the buffers need no barrier.
+    uint32_t scn;
+    float4 rcn;
+    // lane (slot lane / 8, index kj); empty slots take point 0 of the window
+    // with weight 0
+    // (branch-free: slots at or past nslot read clamped addresses and come
+    // out empty, so a chunk's values and loads can be issued unconditionally
+    // between its products)
+    auto load = [&](int c0) {
+      const int q = c0 + (lane >> 3);
+      const int praw = (int)perm[min(q, MAXSLOT - 1)];
+      const bool empty = q >= nslot || praw >= wn;
+      const int pq = empty ? 0 : praw;
+      const uint32_t sc = wsc[pq];
+      rcn = wrec[pq];
+      scn = empty ? 0xFFFFFFFFu : sc;
+    };
+    auto values = [&](float* rb) {
+      float* rr = rb + (lane >> 3) * HM_RS;
+      const float4 r = rcn;
+      const bool empty = scn == 0xFFFFFFFFu;
+      const int dx = rel(scn, 0) & 7;
+      const int dy = rel(scn, 1) & 7;
+      const bool on = kj < a.w;
+      float wx = es_phi<float>((float(kj) - r.x) * a.k.two_over_w, a.k);
+      float wy = es_phi<float>((float(kj) - r.y) * a.k.two_over_w, a.k);
+      float wz = es_phi<float>((float(kj) - r.z) * a.k.two_over_w, a.k);
+      wx = on && !empty ? r.w * wx : 0.f;
+      wy = on ? wy : 0.f;
+      wz = on ? wz : 0.f;
+      rr[kj + dy] = wy;
+      rr[(kj + dy + 8) & 15] = 0.f;
+      rr[16 + kj + dx] = wx;
+      rr[16 + ((kj + dx + 8) & 15)] = 0.f;
+      rr[32 + kj] = wz;
+    };
+    // key run bounds (slots), uniform
+    int kb[HM_NK + 1];
+    kb[0] = 0;
+#pragma unroll
+    for (int K = 0; K < HM_NK; ++K) kb[K + 1] = __builtin_amdgcn_readfirstlane(kend[K]);
+    load(0);
+    values(recs[0]);
+    load(HM_CH);
+    int c0 = 0, cb = 0;
+    hm_static_for<0, HM_NK>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      for (; c0 < kb[K + 1]; c0 += HM_CH, cb ^= 1) {
+        const float* r0 = recs[cb] + k4 * HM_RS;
+        const float* r1 = r0 + 4 * HM_RS;
+        const float av0 = r0[i16], bx0 = r0[16 + i16];
+        const float4 zl0 = *reinterpret_cast<const float4*>(r0 + 32), zh0 = *reinterpret_cast<const float4*>(r0 + 36);
+        const float av1 = r1[i16], bx1 = r1[16 + i16];
+        const float4 zl1 = *reinterpret_cast<const float4*>(r1 + 32), zh1 = *reinterpret_cast<const float4*>(r1 + 36);
+        values(recs[cb ^ 1]);
+        load(c0 + 2 * HM_CH);
+        hm_products<K>(acc, av0, bx0, zl0, zh0);
+        hm_products<K>(acc, av1, bx1, zl1, zh1);
+      }
+    });
+    __syncthreads();
+  }
+  // out: register r of lane (k4, i16) holds row y = 4 k4 + r, column
+  // x = 8 h + i16.  The tile's other 8 columns and its row 15 (beyond the
+  // 8 + 7 rows a bin's points reach) are zero and not written: the buffer
+  // holds zeros there from its last fill.
+  const int xo = 8 * h + i16;
+#pragma unroll
+  for (int z = 0; z < HP_Z; ++z)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < 3 || k4 < 3) out[(z * HS_Y + 4 * k4 + r) * HS_X + xo] = acc[z][r];
+}
+
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
 // covering fine-grid coordinate c along dimension a, -1 terminated
 template <typename T, int TY>
@@ -2008,8 +2250,11 @@ __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__
 // dimension (every cell unless a partial last tile sits next to the wrap)
 // issues all its up to 8 P halo loads at once and sums them in the same order
 // as k_merge_halo (z, y, x entries, then slices), so the grid is bitwise the
-// same; 32-bit offsets (the host checks the halo buffer size).
-template <typename T, int TY, int P>
+// same; 32-bit offsets (the host checks the halo buffer size).  WIN: the
+// parts alternate x windows [0, 16) and [8, 24) of the tile (the matrix-core
+// spreader), zero outside them, so a part is read only where its window
+// covers the cell (the terms skipped are exact zeros).
+template <typename T, int TY, int P, bool WIN = false>
 __global__ void __launch_bounds__(256)
     k_merge_halo_p(const T* __restrict__ halo, const int* __restrict__ tabs, SpreadDev<T> a, T* __restrict__ grid) {
   constexpr int HS_Y = HaloShape<TY>::HS_Y;
@@ -2042,7 +2287,10 @@ __global__ void __launch_bounds__(256)
         const bool ok = (ix == 0 || vx1) && (iy == 0 || vy1) && (iz == 0 || vz1);
         const int base = (bx[ix] + by[iy] + bz[iz]) * (P * HT) + ux[ix] + uy[iy] + uz[iz];
 #pragma unroll
-        for (int sl = 0; sl < P; ++sl) v[c][sl] = ok ? halo[base + sl * HT] : T(0);
+        for (int sl = 0; sl < P; ++sl) {
+          const bool in = !WIN || ((sl & 1) ? ux[ix] >= 8 : ux[ix] < 16);
+          v[c][sl] = ok && in ? halo[base + sl * HT] : T(0);
+        }
       }
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -2229,7 +2477,21 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
 #define FRESCO_HALO_MW(TY_, CH_)                                                                          \
   hipLaunchKernelGGL((k_spread3_halo_mw<T, TY_, 4, CH_>), grd, dim3(256), 0, s, p.scell.get(), p.rec.get(), \
                      p.bin_start.get(), a, sp.halo_buf.get())
-    if (sp.mode == Spreader<T>::HALO4) {
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (sp.mode == Spreader<T>::HALOM && t8) {
+        if (!sp.halo_buf_clear) {
+          fill_zero(sp.halo_buf.get(), sp.halo_buf.size(), s);
+          sp.halo_buf_clear = true;
+        }
+        hipLaunchKernelGGL(k_spread3_halo_mfma, grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
+                           a, sp.halo_buf.get());
+        done = true;
+      }
+    }
+    if (!done) sp.halo_buf_clear = false;  // whole tiles written below
+    if (done) {
+    } else if (sp.mode == Spreader<T>::HALO4) {
       if (t8)
         FRESCO_HALO_MW(8, 32);
       else
@@ -2252,6 +2514,16 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
 #define FRESCO_MERGE_P(TY_, P_)                                                                              \
   hipLaunchKernelGGL((k_merge_halo_p<T, TY_, P_>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, \
                      grid)
+    if (sp.mode == Spreader<T>::HALOM && t8 && (sp.halo_parts == 2 || sp.halo_parts == 4)) {
+      if (sp.halo_parts == 2)
+        hipLaunchKernelGGL((k_merge_halo_p<T, 8, 2, true>), grd, dim3(256), 0, s, sp.halo_buf.get(),
+                           sp.halo_tab.get(), a, grid);
+      else
+        hipLaunchKernelGGL((k_merge_halo_p<T, 8, 4, true>), grd, dim3(256), 0, s, sp.halo_buf.get(),
+                           sp.halo_tab.get(), a, grid);
+      FRESCO_HIP_CHECK(hipGetLastError());
+      return true;
+    }
     switch (sp.halo_parts * (t8 ? 1 : -1)) {
       case 1: FRESCO_MERGE_P(8, 1); break;
       case 2: FRESCO_MERGE_P(8, 2); break;

@@ -258,6 +258,50 @@ def test_halo_spreader_wavefront_layouts_agree(NUwU, prec, eps, tol, monkeypatch
         assert np.array_equal(res["halo:" + sl][1], res["halo4:" + sl][1])
 
 
+def _clustered(N, frac, seed):
+    """N points, a fraction frac of them in a cube of side 0.04 (a few bins
+    then hold hundreds to thousands of points, most bins none)."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0, 1, (N, 3))
+    nc = int(frac * N)
+    p[:nc] = 0.3 + 0.04 * rng.uniform(0, 1, (nc, 3))
+    return p.ravel()
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered"])
+def test_matrix_core_spreader(NUwU, case, monkeypatch):
+    """The FP32 halo spreading as matrix products (points ranked by the z of
+    their first cell in each x window, the window's planes accumulated in
+    registers), at one and two slices per bin: parity with the exact sums,
+    agreement with the LDS halo spreader to FP32 rounding, bitwise repeats.
+    The clustered case puts over a thousand points into single bins (several
+    ranking windows per slice, runs padded to whole chunks) and leaves most
+    bins empty."""
+    n, N = 21, 6000
+    radii, Sk, V, p = make_case(3, n, N, seed=81)
+    if case == "clustered":
+        p = _clustered(N, 0.7, seed=82)
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
+    E0, g0 = ref.get_energy_gradient(p)
+    res = {}
+    for mode in ("halom:2", "halom:4", "halo4:2"):
+        monkeypatch.setenv("FRESCO_SPREAD3", mode)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        info = pot.plan_info()
+        print(case, mode, info, flush=True)
+        assert info["spreader"] == mode.split(":")[0] and info["spread_parts"] == int(mode[-1])
+        E, g = pot.get_energy_gradient(p)
+        assert abs(E - E0) / abs(E0) < 1e-5
+        assert rel(g, g0) < 1e-5
+        assert rel(pot.get_rho(), ref.rho(p)) < 1e-5
+        E2, g2 = pot.get_energy_gradient(p)
+        assert E == E2 and np.array_equal(g, g2)
+        res[mode] = (E, g)
+    for mode in ("halom:2", "halom:4"):
+        assert abs(res[mode][0] - res["halo4:2"][0]) / abs(E0) < 2e-6
+        assert rel(res[mode][1], res["halo4:2"][1]) < 2e-6
+
+
 @pytest.mark.parametrize("d,n,N", [(3, 33, 5000), (3, 21, 2047), (2, 65, 4097), (1, 129, 3000)])
 def test_direct_binning_equals_radix_sort(NUwU, d, n, N, monkeypatch):
     """Plans whose bins fit one 11-bit digit are binned by one counting pass
