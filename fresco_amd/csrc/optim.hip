@@ -89,9 +89,10 @@ __global__ void k_axpy_dev(double* __restrict__ y, const double* __restrict__ x,
     y[i] += c * x[i];
 }
 
-// y = a * x + b * z
+// y = a * x + b * z   (b = *bdev when bdev is given: a device scalar)
 __global__ void k_axpbz(double* __restrict__ y, double a, const double* __restrict__ x, double b,
-                        const double* __restrict__ z, int64_t n) {
+                        const double* __restrict__ z, int64_t n, const double* __restrict__ bdev = nullptr) {
+  if (bdev) b = *bdev;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i] + (z ? b * z[i] : 0.0);
 }
@@ -119,11 +120,39 @@ __global__ void __launch_bounds__(RB)
   }
 }
 
+// FIRE device scalars (the recorded iteration keeps its state here)
+enum : int {
+  FS_F = 0,   // energy at the current x
+  FS_FOLD,    // energy at the previous x
+  FS_DT,      // time step
+  FS_A,       // mixing
+  FS_IFNORM,  // 1 / |g|
+  FS_VNORM,   // |v|
+  FS_NITER,   // iterations since the last uphill step
+  FS_P,       // power -v . g of the last step
+  FS_ND,      // |dx|^2
+  FS_FACTOR,  // step scale (maxstep)
+  FS_UPHILL,  // 1 when the last step went uphill (P <= 0)
+  FS_E,       // energy of the last evaluation
+  FS_R0,      // v . g, g . g, v . v at the new x
+  FS_R1,
+  FS_R2,
+  FS_RMS,
+  FS_COUNT
+};
+
 // FIRE: v = (1-a) v - a g ifnorm vnorm ; v -= dt g ; dx = dt v ; partial |dx|^2
+// (a, ifnorm vnorm and dt from dsc when it is given)
 __global__ void __launch_bounds__(RB)
     k_fire_euler(double* __restrict__ v, const double* __restrict__ g, double* __restrict__ dx, double a,
-                 double gscale, double dt, int64_t n, double* __restrict__ part) {
+                 double gscale, double dt, int64_t n, double* __restrict__ part,
+                 const double* __restrict__ dsc = nullptr) {
   __shared__ double sh[RB];
+  if (dsc) {
+    a = dsc[FS_A];
+    gscale = dsc[FS_IFNORM] * dsc[FS_VNORM];
+    dt = dsc[FS_DT];
+  }
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
     double vi = (1.0 - a) * v[i] - a * g[i] * gscale;  // modified_fire.hpp:172-174
@@ -135,6 +164,54 @@ __global__ void __launch_bounds__(RB)
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// modified_fire.hpp:146-158: step scale from |dx| (FS_ND); the iteration's
+// bookkeeping (previous energy, iteration count)
+__global__ void k_fire_factor(double* __restrict__ dsc, double fmaxstep) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    dsc[FS_FOLD] = dsc[FS_F];
+    dsc[FS_NITER] += 1.0;
+    const double normdx = sqrt(dsc[FS_ND]);
+    double factor = 1;
+    if (normdx > fmaxstep) factor = fmaxstep / normdx;
+    dsc[FS_FACTOR] = factor;
+  }
+}
+
+// modified_fire.hpp:178-216 after the evaluation: FS_R0..2 = v.g, g.g, v.v
+__global__ void k_fire_update(double* __restrict__ dsc, double Nmin, double finc, double dtmax, double fa,
+                              double fdec, double astart, double sqrtn, int stepback) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double P = -dsc[FS_R0];
+    dsc[FS_P] = P;
+    dsc[FS_F] = dsc[FS_E];
+    if (P > 0) {
+      if (dsc[FS_NITER] > Nmin) {
+        const double dtn = dsc[FS_DT] * finc;
+        dsc[FS_DT] = dtmax < dtn ? dtmax : dtn;  // std::min order
+        dsc[FS_A] *= fa;
+      }
+      const double ifn = 1. / sqrt(dsc[FS_R1]);
+      dsc[FS_IFNORM] = ifn;
+      dsc[FS_VNORM] = sqrt(dsc[FS_R2]);
+      dsc[FS_RMS] = 1. / (ifn * sqrtn);
+      dsc[FS_UPHILL] = 0.0;
+    } else {
+      dsc[FS_DT] *= fdec;
+      dsc[FS_A] = astart;
+      dsc[FS_NITER] = 0.0;
+      dsc[FS_UPHILL] = 1.0;
+      if (stepback) dsc[FS_F] = dsc[FS_FOLD];
+    }
+  }
+}
+
+// v = 0 after an uphill step
+__global__ void k_fire_zero_v(double* __restrict__ v, const double* __restrict__ dsc, int64_t n) {
+  if (dsc[FS_UPHILL] == 0.0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0;
 }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(RMAXB, (n + RB - 1) / RB)); }
@@ -1261,9 +1338,113 @@ ModifiedFire::ModifiedFire(std::shared_ptr<Potential> pot, const double* x0, int
   dx_.allocate(n);
   xold_.allocate(n);
   gold_.allocate(n);
+  dsc_.allocate(FS_COUNT);
+  FRESCO_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hsc_), FS_COUNT * sizeof(double)));
   ops_.zero(v_.get(), stream_);
   ops_.copy(xold_.get(), x_.get(), stream_);
   ops_.zero(gold_.get(), stream_);
+}
+
+ModifiedFire::~ModifiedFire() {
+  for (int q = 0; q < 2; ++q)
+    if (gexec_[q]) (void)hipGraphExecDestroy(gexec_[q]);
+  if (cap_) (void)hipStreamDestroy(cap_);
+  if (hsc_) (void)hipHostFree(hsc_);
+}
+
+// The iteration below as a fixed kernel sequence on the device scalars: the
+// new x goes to the other position buffer and its gradient to the other
+// gradient buffer (the current ones become x_old, g_old), so one recorded
+// graph per assignment of the two buffer pairs.
+void ModifiedFire::enqueue_iteration(hipStream_t s) {
+  double* dsc = dsc_.get();
+  const unsigned nb = nblocks(n_);
+  hipLaunchKernelGGL(k_fire_euler, dim3(nb), dim3(RB), 0, s, v_.get(), g_.get(), dx_.get(), 0.0, 0.0, 0.0, n_,
+                     ops_.part.get(), dsc);
+  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(RB), 0, s, ops_.part.get(), (int)nb, 1, 1.0, nullptr, dsc + FS_ND);
+  hipLaunchKernelGGL(k_fire_factor, dim3(1), dim3(64), 0, s, dsc, fmaxstep_);
+  hipLaunchKernelGGL(k_axpbz, dim3(eblocks(n_)), dim3(256), 0, s, xold_.get(), 1.0, x_.get(), 0.0, dx_.get(), n_,
+                     dsc + FS_FACTOR);
+  FRESCO_HIP_CHECK(hipGetLastError());
+  if (!pot_->energy_gradient_async(xold_.get(), gold_.get(), dsc + FS_E, s))
+    throw RuntimeFailure("potential has no device-side evaluation");
+  hipLaunchKernelGGL(k_dots, dim3(nb), dim3(RB), 0, s, v_.get(), gold_.get(), gold_.get(), gold_.get(), v_.get(),
+                     v_.get(), n_, 3, ops_.part.get());
+  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(RB), 0, s, ops_.part.get(), (int)nb, 3, 1.0, nullptr, dsc + FS_R0);
+  hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(64), 0, s, dsc, Nmin_, finc_, dtmax_, fa_, fdec_, astart_,
+                     std::sqrt((double)n_), stepback_ ? 1 : 0);
+  hipLaunchKernelGGL(k_fire_zero_v, dim3(eblocks(n_)), dim3(256), 0, s, v_.get(), dsc, n_);
+  FRESCO_HIP_CHECK(hipGetLastError());
+}
+
+void ModifiedFire::upload_scalars() {
+  hsc_[FS_F] = f_;
+  hsc_[FS_FOLD] = fold_;
+  hsc_[FS_DT] = dt_;
+  hsc_[FS_A] = a_;
+  hsc_[FS_IFNORM] = ifnorm_;
+  hsc_[FS_VNORM] = vnorm_;
+  hsc_[FS_NITER] = (double)fire_iter_number_;
+  hsc_[FS_RMS] = rms_;
+  FRESCO_HIP_CHECK(hipMemcpyAsync(dsc_.get(), hsc_, FS_COUNT * sizeof(double), hipMemcpyHostToDevice, stream_));
+  dirty_ = false;
+}
+
+void ModifiedFire::graph_iteration() {
+  if (dirty_) upload_scalars();
+  const uint64_t ep = pot_->graph_epoch();
+  if (ep != gepoch_) {
+    for (int q = 0; q < 2; ++q) {
+      if (gexec_[q]) FRESCO_HIP_CHECK(hipGraphExecDestroy(gexec_[q]));
+      gexec_[q] = nullptr;
+      gx_[q] = nullptr;
+    }
+    gepoch_ = ep;
+  }
+  int q = gx_[0] == x_.get() ? 0 : (gx_[1] == x_.get() ? 1 : -1);
+  if (q < 0 || !gexec_[q]) {
+    q = gexec_[0] ? 1 : 0;
+    if (!cap_) FRESCO_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    FRESCO_HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_iteration(cap_);
+    } catch (...) {
+      (void)hipStreamEndCapture(cap_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    FRESCO_HIP_CHECK(hipStreamEndCapture(cap_, &g));
+    const hipError_t e = hipGraphInstantiate(&gexec_[q], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FRESCO_HIP_CHECK(e);
+    gx_[q] = x_.get();
+    ++captures_;
+  }
+  FRESCO_HIP_CHECK(hipGraphLaunch(gexec_[q], stream_));
+  ++replays_;
+  FRESCO_HIP_CHECK(hipMemcpyAsync(hsc_, dsc_.get(), FS_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  FRESCO_HIP_CHECK(hipStreamSynchronize(stream_));
+  f_ = hsc_[FS_F];
+  fold_ = hsc_[FS_FOLD];
+  dt_ = hsc_[FS_DT];
+  a_ = hsc_[FS_A];
+  ifnorm_ = hsc_[FS_IFNORM];
+  vnorm_ = hsc_[FS_VNORM];
+  fire_iter_number_ = (size_t)hsc_[FS_NITER];
+  rms_ = hsc_[FS_RMS];
+  const bool uphill = hsc_[FS_UPHILL] != 0.0;
+  if (!(uphill && stepback_)) {
+    // the new x, g were written to the other pair of buffers
+    std::swap(x_, xold_);
+    std::swap(g_, gold_);
+  }
+  if (uphill && verbosity_ > 1) std::cout << "warning: step direction was uphill.  inverting\n";
+  if (iprint_ > 0 && iter_number_ % iprint_ == 0) {
+    std::cout << "fire: " << iter_number_ << " fire_iter_number " << fire_iter_number_ << " dt " << dt_ << " a " << a_
+              << " P " << hsc_[FS_P] << " vnorm " << vnorm_ << " E " << f_ << " rms " << rms_ << " nfev " << nfev_
+              << "\n";
+  }
 }
 
 void ModifiedFire::initialize_func_gradient() {
@@ -1277,6 +1458,7 @@ void ModifiedFire::initialize_func_gradient() {
   vnorm_ = std::sqrt(r[1]);
   rms_ = 1. / (ifnorm_ * std::sqrt((double)n_));
   func_initialized_ = true;
+  dirty_ = true;
 }
 
 void ModifiedFire::set_func_gradient(double f, const double* grad_host) {
@@ -1291,12 +1473,18 @@ void ModifiedFire::set_func_gradient(double f, const double* grad_host) {
   vnorm_ = std::sqrt(r[1]);
   rms_ = 1. / (ifnorm_ * std::sqrt((double)n_));
   func_initialized_ = true;
+  dirty_ = true;
 }
 
 void ModifiedFire::one_iteration() {
   if (!func_initialized_) initialize_func_gradient();
   nfev_ += 1;
   iter_number_ += 1;
+  if (use_graph_ && pot_->capture_ready() && !prof_enabled()) {
+    graph_iteration();
+    return;
+  }
+  dirty_ = true;
   fire_iter_number_ += 1;
   fold_ = f_;
   // x_old = x, g_old = g (modified_fire.hpp:146, 168) by exchanging buffer
@@ -1367,6 +1555,7 @@ void ModifiedFire::reset(const double* x0_host) {
   ifnorm_ = 1. / std::sqrt(r[0]);
   vnorm_ = std::sqrt(r[1]);
   rms_ = 1. / (ifnorm_ * std::sqrt((double)n_));
+  dirty_ = true;
 }
 
 }  // namespace fresco_hip

@@ -157,17 +157,39 @@ class ModifiedFire final : public GradientOptimizer {
   ModifiedFire(std::shared_ptr<Potential> pot, const double* x0, int64_t n, double dtstart, double dtmax,
                double maxstep, int64_t Nmin, double finc, double fdec, double fa, double astart, double tol,
                bool stepback);
+  ~ModifiedFire() override;
+  ModifiedFire(const ModifiedFire&) = delete;
+  ModifiedFire& operator=(const ModifiedFire&) = delete;
   void one_iteration() override;
   void reset(const double* x0_host) override;
   void initialize_func_gradient() override;
   void set_func_gradient(double f, const double* grad_host) override;
+  void set_use_graph(bool on) override { use_graph_ = on; }
+  void graph_stats(int64_t* captures, int64_t* replays) const override {
+    *captures = captures_;
+    *replays = replays_;
+  }
 
  private:
+  // iterations on device scalars, recorded as HIP graphs (one per buffer
+  // assignment) when the potential can be recorded
+  void enqueue_iteration(hipStream_t s);
+  void graph_iteration();
+  void upload_scalars();
+
   double dtstart_, dt_, dtmax_, fmaxstep_, Nmin_, finc_, fdec_, fa_, astart_, a_;
   double fold_ = 0, ifnorm_ = 0, vnorm_ = 0;
   size_t fire_iter_number_ = 0;
   bool stepback_;
-  DeviceBuffer<double> v_, dx_, xold_, gold_;
+  DeviceBuffer<double> v_, dx_, xold_, gold_, dsc_;
+  double* hsc_ = nullptr;  // pinned mirror of the device scalars
+  bool dirty_ = true;      // host scalars newer than the device ones
+  bool use_graph_ = true;
+  hipStream_t cap_ = nullptr;
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+  const double* gx_[2] = {nullptr, nullptr};  // x buffer each graph starts from
+  uint64_t gepoch_ = 0;
+  int64_t captures_ = 0, replays_ = 0;
 };
 
 }  // namespace fresco_hip

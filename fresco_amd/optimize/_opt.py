@@ -115,6 +115,12 @@ class GradientOptimizer:
         res.success = self.stop_criterion_satisfied()
         return res
 
+    def graph_stats(self):
+        """(graphs recorded, iterations replayed from a graph)."""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.load().fresco_opt_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return int(c.value), int(r.value)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -172,12 +178,6 @@ class LBFGS_CPP(GradientOptimizer):
     def set_H0(self, H0):
         self._set(_lib.OPT_H0, float(H0))
 
-    def graph_stats(self):
-        """(graphs recorded, iterations replayed from a graph)."""
-        c, r = ctypes.c_int64(), ctypes.c_int64()
-        _lib.check(_lib.load().fresco_opt_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
-        return int(c.value), int(r.value)
-
     def get_result(self):
         res = super().get_result()
         res["H0"] = float(self._state()[5])
@@ -185,11 +185,17 @@ class LBFGS_CPP(GradientOptimizer):
 
 
 class ModifiedFireCPP(GradientOptimizer):
-    """Modified FIRE on the GPU (modified_fire.hpp)."""
+    """Modified FIRE on the GPU (modified_fire.hpp).
+
+    use_graph (not in the reference): keep the iteration's scalars (dt, the
+    mixing, the norms, the step count) on the device and replay each
+    iteration as a recorded HIP graph when the potential can be recorded;
+    False runs the host-controlled iteration, with identical results.
+    """
 
     def __init__(self, x0, potential, dtstart=0.1, dtmax=1, maxstep=0.5, Nmin=5, finc=1.1, fdec=0.5, fa=0.99,
                  astart=0.1, tol=1e-3, stepback=True, iprint=-1, energy=None, gradient=None, nsteps=10000,
-                 verbosity=0, events=None):
+                 verbosity=0, events=None, use_graph=True):
         x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).ravel())
         self._n = len(x)
         self.pot = as_cpp_potential(potential, verbose=verbosity > 0, ndof=self._n)
@@ -204,6 +210,7 @@ class ModifiedFireCPP(GradientOptimizer):
         self._set(_lib.OPT_MAX_ITER, nsteps)
         self._set(_lib.OPT_VERBOSITY, verbosity)
         self._set(_lib.OPT_IPRINT, iprint)
+        self._set(_lib.OPT_USE_GRAPH, 1 if use_graph else 0)
         if energy is not None and gradient is not None:
             g = np.ascontiguousarray(np.asarray(gradient, dtype=np.float64).ravel())
             _lib.check(_lib.load().fresco_opt_set_func_gradient(self._h, float(energy), _lib.dptr(g), len(g)))

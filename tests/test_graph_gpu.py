@@ -167,3 +167,46 @@ def test_graph_off_for_unrecordable_potentials(mods):
     for it in range(20):
         same(ga.one_iteration(), eb.one_iteration())
     assert ga.graph_stats() == (0, 0)
+
+
+def _fire_same(ra, rb):
+    assert np.array_equal(ra.coords, rb.coords)
+    assert ra.energy == rb.energy and ra.rms == rb.rms
+    assert ra.nfev == rb.nfev and ra.nsteps == rb.nsteps
+
+
+@pytest.mark.parametrize("name,stepback", [("nuwu2d_fp64", True), ("nuwu2d_fp32", True), ("nuwu3d_fp32", False),
+                                           ("nuwu3d_fp32", True), ("uwu3d_fp32", True), ("batch2d_fp32", True)])
+def test_fire_graph_replays_match_host_iterations(mods, name, stepback):
+    """Modified FIRE with its scalars on the device, each iteration replayed
+    from one of two recorded graphs (the position / gradient buffer pairs
+    swap roles every accepted step), against the host-controlled iteration
+    of modified_fire.hpp:136-216 on the same potential: bitwise equal
+    trajectories.  The large time step makes some steps go uphill (power
+    P <= 0: velocity reset, time step cut, and with stepback the previous
+    position restored)."""
+    from fresco_amd.optimize import ModifiedFireCPP
+
+    _, P = mods
+    pot, x0 = make_case(P, name)
+    kw = dict(dtstart=0.02, dtmax=0.2, maxstep=0.05, Nmin=2, tol=0.0, stepback=stepback)
+    ga = ModifiedFireCPP(x0, pot, use_graph=True, **kw)
+    eb = ModifiedFireCPP(x0, pot, use_graph=False, **kw)
+    xr = np.random.default_rng(2).uniform(0, 1, len(x0))
+    energies = []
+    for it in range(40):
+        ra = ga.one_iteration()
+        pot.get_energy_gradient(xr)  # eager work between replays
+        rb = eb.one_iteration()
+        _fire_same(ra, rb)
+        energies.append(ra.energy)
+    cap, rep = ga.graph_stats()
+    assert rep == 40 and 1 <= cap <= 2
+    assert eb.graph_stats() == (0, 0)
+    rises = int(np.sum(np.diff(energies) > 0))
+    backs = int(np.sum(np.diff(energies) == 0))  # stepback restores the previous energy exactly
+    print(name, "stepback", stepback, "graphs", cap, "energy rises", rises, "steps back", backs, "E", energies[-1],
+          flush=True)
+    if name == "nuwu3d_fp32":
+        # this case has uphill steps: the recorded branch is exercised
+        assert (backs if stepback else rises) >= 1
