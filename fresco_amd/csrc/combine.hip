@@ -21,6 +21,15 @@ __global__ void k_add(double* __restrict__ y, const double* __restrict__ x, int6
     y[i] += x[i];
 }
 
+// e = ((0 + e[0]) + e[1]) + ...: the host loop's order
+__global__ void k_sum_energies(const double* __restrict__ e, int n, double* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double acc = 0.;
+    for (int i = 0; i < n; ++i) acc += e[i];
+    *out = acc;
+  }
+}
+
 __global__ void k_zero(double* __restrict__ y, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = 0.0;
@@ -36,8 +45,33 @@ class Combined final : public Potential {
     for (auto& p : pots_)
       if (p->ndof() != n_) throw InvalidArgument("the gradient has the wrong size");
     tmp_.allocate(n_);
+    ebuf_.allocate(pots_.size());
   }
   int64_t ndof() const override { return n_; }
+  // recordable when every term is; the terms' energies are summed on the
+  // device in the order of energy_gradient
+  bool capture_ready() const override {
+    for (auto& p : pots_)
+      if (!p->capture_ready()) return false;
+    return true;
+  }
+  uint64_t graph_epoch() const override {
+    uint64_t e = 0;
+    for (auto& p : pots_) e = e * 1000003ull + p->graph_epoch();
+    return e;
+  }
+  bool energy_gradient_async(const double* x, double* g, double* e_dev, hipStream_t s) override {
+    if (!capture_ready()) return false;
+    hipLaunchKernelGGL(k_zero, dim3(eb(n_)), dim3(256), 0, s, g, n_);
+    for (size_t i = 0; i < pots_.size(); ++i) {
+      if (!pots_[i]->energy_gradient_async(x, tmp_.get(), ebuf_.get() + i, s))
+        throw RuntimeFailure("a recordable potential refused its device-side evaluation");
+      hipLaunchKernelGGL(k_add, dim3(eb(n_)), dim3(256), 0, s, g, tmp_.get(), n_);
+    }
+    hipLaunchKernelGGL(k_sum_energies, dim3(1), dim3(64), 0, s, ebuf_.get(), (int)pots_.size(), e_dev);
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return true;
+  }
   double energy(const double* x, hipStream_t s) override {
     double e = 0.;
     for (auto& p : pots_) e += p->energy(x, s);
@@ -57,7 +91,7 @@ class Combined final : public Potential {
  private:
   std::vector<std::shared_ptr<Potential>> pots_;
   int64_t n_;
-  DeviceBuffer<double> tmp_;
+  DeviceBuffer<double> tmp_, ebuf_;
 };
 
 class Callback final : public Potential {

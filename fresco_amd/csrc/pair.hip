@@ -9,12 +9,13 @@
 // reference's convention), so no atomics and no second pass over pairs.
 //   all pairs : the j loop streams 256-point tiles through LDS;
 //   cell lists: points are binned into cells of side >= the interaction
-//               cutoff, radix-sorted by cell, and each thread visits the
+//               cutoff, sorted by cell (the stable bin sort of sort.hip, so
+//               the whole evaluation is kernels only and can be recorded
+//               in a HIP graph), and each thread visits the
 //               distinct neighbour cells (3^d stencil, wrapped, deduplicated
 //               for boxes of fewer than 3 cells per side).
 // Energies are summed per block in FP64 and reduced in a fixed order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -368,18 +369,27 @@ class PairPotential final : public Potential {
       xs_.allocate((size_t)n * d);
       rs_.allocate(n);
       bad_.allocate(1);
-      size_t bytes = 0;
-      FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key_.get(), key_s_.get(), idx_in_.get(),
-                                                          idx_.get(), (int)n, 0, bits_for(ncells_)));
-      sort_tmp_.allocate(std::max<size_t>(bytes, 1));
-      sort_bytes_ = bytes;
+      sorter_.init(n, bits_for(ncells_));
     }
   }
 
   int64_t ndof() const override { return n_ * d_; }
 
-  double energy(const double* x, hipStream_t s) override { return run<false>(x, nullptr, s); }
-  double energy_gradient(const double* x, double* grad, hipStream_t s) override { return run<true>(x, grad, s); }
+  double energy(const double* x, hipStream_t s) override {
+    run<false>(x, nullptr, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+  double energy_gradient(const double* x, double* grad, hipStream_t s) override {
+    run<true>(x, grad, e_.get(), s);
+    return host_.fetch(e_.get(), s);
+  }
+  // kernels only (cell keys, bin sort, gather, pairs, fixed-order energy sum)
+  bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
+    run<true>(x, grad, e_dev, s);
+    return true;
+  }
+  bool capture_ready() const override { return true; }
+  uint64_t graph_epoch() const override { return 1; }
 
  private:
   static int bits_for(int ncells) {
@@ -388,34 +398,34 @@ class PairPotential final : public Potential {
     return b;
   }
 
+  // enqueues the evaluation; the energy goes to e_out (device)
   template <bool GRAD>
-  double run(const double* x, double* grad, hipStream_t s) {
+  void run(const double* x, double* grad, double* e_out, hipStream_t s) {
     switch (d_) {
-      case 1: return run_d<1, GRAD>(x, grad, s);
-      case 2: return run_d<2, GRAD>(x, grad, s);
-      case 3: return run_d<3, GRAD>(x, grad, s);
-      default: return run_d<4, GRAD>(x, grad, s);
+      case 1: run_d<1, GRAD>(x, grad, e_out, s); break;
+      case 2: run_d<2, GRAD>(x, grad, e_out, s); break;
+      case 3: run_d<3, GRAD>(x, grad, e_out, s); break;
+      default: run_d<4, GRAD>(x, grad, e_out, s); break;
     }
   }
 
   template <int D, bool GRAD>
-  double run_d(const double* x, double* grad, hipStream_t s) {
-    if (kind_ == 0) return run_k<D, 0, GRAD>(x, grad, s);
-    return run_k<D, 1, GRAD>(x, grad, s);
+  void run_d(const double* x, double* grad, double* e_out, hipStream_t s) {
+    if (kind_ == 0)
+      run_k<D, 0, GRAD>(x, grad, e_out, s);
+    else
+      run_k<D, 1, GRAD>(x, grad, e_out, s);
   }
 
   template <int D, int KIND, bool GRAD>
-  double run_k(const double* x, double* grad, hipStream_t s) {
+  void run_k(const double* x, double* grad, double* e_out, hipStream_t s) {
     if (cells_) {
       const unsigned nb = blocks_for(n_, PT);
       fill_zero(bad_.get(), 1, s);
       hipLaunchKernelGGL(k_cell_keys<D>, dim3(nb), dim3(PT), 0, s, x, n_, b_, c_, key_.get(), idx_in_.get(),
                          bad_.get());
       FRESCO_HIP_CHECK(hipGetLastError());
-      size_t bytes = sort_bytes_;
-      FRESCO_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.get(), bytes, key_.get(), key_s_.get(),
-                                                          idx_in_.get(), idx_.get(), (int)n_, 0, bits_for(ncells_),
-                                                          s));
+      sorter_.sort(key_.get(), idx_in_.get(), key_s_.get(), idx_.get(), s);
       hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)ncells_ + 1, PT)), dim3(PT), 0, s, key_s_.get(), n_,
                          ncells_, start_.get());
       hipLaunchKernelGGL(k_cell_gather<D>, dim3(nb), dim3(PT), 0, s, x, radii_.get(), idx_.get(), n_, xs_.get(),
@@ -432,8 +442,7 @@ class PairPotential final : public Potential {
                          P_, grad, epart_.get());
       FRESCO_HIP_CHECK(hipGetLastError());
     }
-    reduce_partials(epart_.get(), nblk_, e_.get(), s);
-    return host_.fetch(e_.get(), s);
+    reduce_partials(epart_.get(), nblk_, e_out, s);
   }
 
   int kind_;
@@ -449,8 +458,7 @@ class PairPotential final : public Potential {
   DeviceBuffer<uint32_t> key_, key_s_, idx_in_, idx_;
   DeviceBuffer<int> start_;
   DeviceBuffer<unsigned int> bad_;
-  DeviceBuffer<unsigned char> sort_tmp_;
-  size_t sort_bytes_ = 0;
+  BinSort sorter_;
   HostScalar host_;
 };
 

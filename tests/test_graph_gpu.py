@@ -79,6 +79,20 @@ def make_case(P, name):
         pot = P.NUwNU(rng.uniform(0.5, 1.5, N), K.ravel(), Sk, V, np.ones(2), 1e-6, precision="fp32",
                       method=method)
         return pot, rng.uniform(0, 1, 2 * N)
+    if name in ("pair3d_cells", "wca2d_allpairs", "combined2d"):
+        d, N = (3, 2000) if name == "pair3d_cells" else (2, 600)
+        radii = rng.uniform(0.45, 0.55, N)
+        vol = np.sum(np.pi * radii ** 2) if d == 2 else np.sum(4 / 3 * np.pi * radii ** 3)
+        box = np.full(d, (vol / 0.6) ** (1.0 / d))
+        x0 = rng.uniform(0, 1, (N, d)) * box
+        if name == "wca2d_allpairs":
+            return P.WCAPotential(1.0, 1.0, radii, box), x0.ravel()
+        pair = P.InversePower(2.5, 1.0, radii, box, use_cell_lists=True)
+        if name == "pair3d_cells":
+            return pair, x0.ravel()
+        Sk, V = stealthy(8, 2, 4)
+        nuwu = P.NUwU(np.ones(N), 8, Sk, V, box, 1e-12, precision="fp64")
+        return P.CombinedPotential([nuwu, pair]), x0.ravel()
     raise ValueError(name)
 
 
@@ -89,7 +103,8 @@ def same(ra, rb):
 
 
 @pytest.mark.parametrize("name", ["nuwu2d_fp64", "nuwu2d_fp32", "nuwu3d_fp32", "batch2d_fp32", "uwu2d_fp64",
-                                  "uwu3d_fp32", "uwnu2d_fp32", "nuwnu2d_type3", "nuwnu2d_direct"])
+                                  "uwu3d_fp32", "uwnu2d_fp32", "nuwnu2d_type3", "nuwnu2d_direct", "pair3d_cells",
+                                  "wca2d_allpairs", "combined2d"])
 def test_graph_replays_match_launches(mods, name):
     LBFGS_CPP, P = mods
     pot, x0 = make_case(P, name)
@@ -148,9 +163,9 @@ def test_graph_rerecorded_when_targets_grow(mods):
 
 
 def test_graph_off_for_unrecordable_potentials(mods):
-    """Potentials without an enqueue-only evaluation (here a Python one; also
-    pair and combined potentials) run the launch-by-launch path with the same
-    results."""
+    """Potentials without an enqueue-only evaluation (here a Python one, alone
+    and inside a combined potential) run the launch-by-launch path with the
+    same results."""
     LBFGS_CPP, P = mods
 
     class Quartic:
@@ -167,6 +182,16 @@ def test_graph_off_for_unrecordable_potentials(mods):
     for it in range(20):
         same(ga.one_iteration(), eb.one_iteration())
     assert ga.graph_stats() == (0, 0)
+    # a combined potential with a host term is not recorded either
+    box = np.full(2, 6.0)
+    xc = np.random.default_rng(6).uniform(0, 6, 50)
+    comb = P.CombinedPotential([P.InversePower(2.5, 1.0, np.full(25, 0.5), box, use_cell_lists=True),
+                                P.PythonPotential(Quartic(), 50)])
+    gc = LBFGS_CPP(xc, comb, tol=0.0, M=4, use_graph=True)
+    ec = LBFGS_CPP(xc, comb, tol=0.0, M=4, use_graph=False)
+    for it in range(10):
+        same(gc.one_iteration(), ec.one_iteration())
+    assert gc.graph_stats() == (0, 0)
 
 
 def _fire_same(ra, rb):
@@ -176,7 +201,8 @@ def _fire_same(ra, rb):
 
 
 @pytest.mark.parametrize("name,stepback", [("nuwu2d_fp64", True), ("nuwu2d_fp32", True), ("nuwu3d_fp32", False),
-                                           ("nuwu3d_fp32", True), ("uwu3d_fp32", True), ("batch2d_fp32", True)])
+                                           ("nuwu3d_fp32", True), ("uwu3d_fp32", True), ("batch2d_fp32", True),
+                                           ("pair3d_cells", True), ("combined2d", True)])
 def test_fire_graph_replays_match_host_iterations(mods, name, stepback):
     """Modified FIRE with its scalars on the device, each iteration replayed
     from one of two recorded graphs (the position / gradient buffer pairs
