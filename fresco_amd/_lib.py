@@ -72,6 +72,15 @@ SIGNATURES = {
     ),
     "fresco_pot_get_rho": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fresco_pot_energy_keep": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+         ctypes.c_void_p],
+    ),
+    "fresco_pot_gradient_kept": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
+    ),
     "fresco_pot_set_targets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fresco_pot_spread_config": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     "fresco_pot_spread_source": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),

@@ -350,6 +350,23 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad, d
   });
 }
 
+int fresco_pot_energy_keep(fresco_pot* pot, const double* x, double* energy, uint64_t* token, void* stream) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!x || !energy || !token) throw fresco_hip::InvalidArgument("null pointer");
+    uint64_t t = 0;
+    *token = p->energy_keep(x, energy, &t, static_cast<hipStream_t>(stream)) ? t : 0;
+  });
+}
+
+int fresco_pot_gradient_kept(fresco_pot* pot, uint64_t token, double* grad, int* ok, void* stream) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!ok) throw fresco_hip::InvalidArgument("null pointer");
+    *ok = token != 0 && p->gradient_kept(token, grad, static_cast<hipStream_t>(stream)) ? 1 : 0;
+  });
+}
+
 int fresco_pot_get_rho(fresco_pot* pot, double* rho, void* stream) {
   return guarded([&]() {
     auto* p = checked(pot);
@@ -369,7 +386,7 @@ int fresco_pot_backward(fresco_pot* pot, double* dS0, double* dV, void* stream) 
 int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, void* stream) {
   return guarded([&]() {
     auto* p = checked(pot);
-    if (!Sk || !V) throw fresco_hip::InvalidArgument("null pointer");
+    if (!Sk) throw fresco_hip::InvalidArgument("null pointer");
     p->set_targets(Sk, V, static_cast<hipStream_t>(stream));
   });
 }

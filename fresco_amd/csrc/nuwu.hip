@@ -256,6 +256,7 @@ class NUwU final : public Potential {
 
   double energy(const double* x, hipStream_t s) override {
     last_x_ = x;
+    kept_ = false;
     {
       ProfScope p(ST_SORT, s);
       pts_.set(x, map_e_, c_.get(), s);
@@ -279,6 +280,23 @@ class NUwU final : public Potential {
 
   bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
     enqueue_eg(x, grad, e_dev, nullptr, s);
+    return true;
+  }
+
+  // the gradient pass split at the pullback spectrum: spreading, transform
+  // and loss now, the inverse transform and interpolation in gradient_kept
+  // (the sorted points and the pullback spectrum stay until the next
+  // evaluation or target update)
+  bool energy_keep(const double* x, double* e, uint64_t* token, hipStream_t s) override {
+    enqueue_front(x, e_dev_.get(), host_.dev(), s);
+    *e = host_.wait(s);
+    kept_ = true;
+    *token = ++keep_serial_;
+    return true;
+  }
+  bool gradient_kept(uint64_t token, double* grad, hipStream_t s) override {
+    if (!kept_ || token != keep_serial_) return false;
+    if (grad) enqueue_back(grad, s);
     return true;
   }
   // the spreader's first call may time its candidates (host waits)
@@ -335,7 +353,13 @@ class NUwU final : public Potential {
   }
 
   void set_targets(const double* Sk, const double* V, hipStream_t s) override {
+    kept_ = false;
     FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_full_.get(), Sk, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (!V) {
+      // weights unchanged: the active modes stay, no host check
+      regather_targets(s);
+      return;
+    }
     FRESCO_HIP_CHECK(hipMemcpyAsync(V_full_.get(), V, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
     // modes switched on outside the active pair set need new pair tables
     fill_zero(count_.get(), 1, s);
@@ -354,6 +378,12 @@ class NUwU final : public Potential {
       build_pairs(hs.data(), hv.data());
       return;
     }
+    regather_targets(s);
+  }
+
+ private:
+  // the pair tables' targets and weights from the full arrays
+  void regather_targets(hipStream_t s) {
     if (npairs_ > 0) {
       hipLaunchKernelGGL(k_gather_targets, dim3((npairs_ + 255) / 256), dim3(256), 0, s, Sk_full_.get(),
                          V_full_.get(), pd_, npairs_, p_vk_.get(), p_vm_.get(), p_sk_.get(), p_sm_.get());
@@ -361,10 +391,16 @@ class NUwU final : public Potential {
     }
   }
 
- private:
   // e_host: device-visible pinned host memory that also receives the energy
   // (the host-synchronous entry points), or null (device-only: graphs)
   void enqueue_eg(const double* x, double* grad, double* e_out, double* e_host, hipStream_t s) {
+    enqueue_front(x, e_out, e_host, s);
+    kept_ = false;
+    enqueue_back(grad, s);
+  }
+
+  // binning, spreading, forward transform, loss and pullback spectrum
+  void enqueue_front(const double* x, double* e_out, double* e_host, hipStream_t s) {
     last_x_ = x;
     {
       ProfScope p(ST_SORT, s);
@@ -390,6 +426,11 @@ class NUwU final : public Potential {
       else
         launch_loss<1>(s, e_out, e_host);
     }
+  }
+
+  // inverse transform(s) of the pullback spectrum and the interpolation of
+  // the gradient at the sorted points
+  void enqueue_back(double* grad, hipStream_t s) {
     if (plain_) {
       for (int a = 0; a < d_; ++a) {
         {
@@ -603,6 +644,8 @@ class NUwU final : public Potential {
   int band_ = 0;                          // modes |k_a| <= band_ are on the fine grid
   std::vector<double> radii_host_;
   const double* last_x_ = nullptr;        // coordinates of the last evaluation
+  bool kept_ = false;                     // energy_keep's state still current
+  uint64_t keep_serial_ = 0;              // token of the last energy_keep
   std::unique_ptr<NUwU<T>> full_;         // full-mode-grid evaluator for get_rho
 };
 

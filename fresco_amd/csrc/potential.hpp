@@ -51,7 +51,27 @@ class Potential {
     (void)s;
     throw RuntimeFailure("this potential has no S0 / V targets");
   }
-  // replace the targets S0 and weights V (device, nmodes each)
+  // energy at x, keeping what gradient_kept needs (the split evaluation of
+  // a differentiable loss: forward here, backward later); *token names the
+  // kept evaluation.  False (nothing done) when the potential has no split
+  virtual bool energy_keep(const double* x, double* e, uint64_t* token, hipStream_t s) {
+    (void)x;
+    (void)e;
+    (void)token;
+    (void)s;
+    return false;
+  }
+  // the gradient (device, ndof) of the kept evaluation named token; false
+  // when that is not what the potential keeps (another evaluation or a
+  // target update came in between).  grad null: only that check
+  virtual bool gradient_kept(uint64_t token, double* grad, hipStream_t s) {
+    (void)token;
+    (void)grad;
+    (void)s;
+    return false;
+  }
+  // replace the targets S0 and weights V (device, nmodes each; V null: the
+  // weights stay)
   virtual void set_targets(const double* Sk, const double* V, hipStream_t s) {
     (void)Sk;
     (void)V;

@@ -194,7 +194,7 @@ class UwU final : public Potential {
 
   void set_targets(const double* Sk, const double* V, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
-    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (V) FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, M_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
  private:
@@ -392,7 +392,7 @@ class UwNU final : public Potential {
 
   void set_targets(const double* Sk, const double* V, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
-    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (V) FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
   void plan_info(int64_t* nf, int* w, double* beta, int* tile) const override {
@@ -632,7 +632,7 @@ class NUwNUDirect final : public Potential {
 
   void set_targets(const double* Sk, const double* V, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
-    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (V) FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
  private:
@@ -957,7 +957,7 @@ class NUwNUType3 final : public Potential {
 
   void set_targets(const double* Sk, const double* V, hipStream_t s) override {
     FRESCO_HIP_CHECK(hipMemcpyAsync(Sk_.get(), Sk, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
-    FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (V) FRESCO_HIP_CHECK(hipMemcpyAsync(V_.get(), V, nk_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
 
   void plan_info(int64_t* nf, int* w, double* beta, int* tile) const override {

@@ -140,33 +140,76 @@ class BasePotential:
         nuwu.hpp:294-314)."""
         raise NotImplementedError
 
-    def set_targets(self, S0, V):
-        """Replace S0(k) and V(k) (same layout as the constructor's Sk / V;
-        numpy arrays or tensors)."""
+    def set_targets(self, S0, V=None):
+        """Replace S0(k) and, unless V is None, V(k) (same layout as the
+        constructor's Sk / V; numpy arrays or tensors).  With V None the
+        weights stay and the update needs no host synchronisation."""
         S0 = torch.as_tensor(S0, dtype=torch.float64)
-        V = torch.as_tensor(V, dtype=torch.float64)
-        s, v = self._targets_flat(S0, V)
+        keep_v = V is None
+        if keep_v:
+            user = getattr(self, "_user_targets", None)
+            Vu = user[1] if user is not None else torch.as_tensor(self.V, dtype=torch.float64)
+        else:
+            Vu = torch.as_tensor(V, dtype=torch.float64)
+        s, v = self._targets_flat(S0, Vu)
         s = s[: self._nmodes].to(device=device(), dtype=torch.float64).contiguous()
-        v = v[: self._nmodes].to(device=device(), dtype=torch.float64).contiguous()
-        if s.numel() != self._nmodes or v.numel() != self._nmodes:
+        if s.numel() != self._nmodes:
             raise ValueError("S0 / V must hold %d modes" % self._nmodes)
+        if keep_v and getattr(self, "_dev_targets", None) is not None:
+            v = self._dev_targets[1]
+        else:
+            v = v[: self._nmodes].to(device=device(), dtype=torch.float64).contiguous()
+            if v.numel() != self._nmodes:
+                raise ValueError("S0 / V must hold %d modes" % self._nmodes)
         _lib.check(
             _lib.load().fresco_pot_set_targets(
-                self._h, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(v.data_ptr()), stream_handle()
+                self._h, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(None if keep_v else v.data_ptr()),
+                stream_handle(),
             )
         )
         self._dev_targets = (s, v)
+        self._user_targets = (S0, Vu)
+        self._ag_targets = None  # the autograd function's record of what it last set
 
     def targets(self):
         """Current (S0, V) in the constructor's layout (numpy)."""
         return self._current("Sk"), self._current("V")
 
     def _current(self, name):
-        """Constructor-order Sk / V as numpy, including set_targets updates."""
-        dev = getattr(self, "_dev_targets", None)
-        if dev is None:
+        """Constructor-layout Sk / V as numpy, including set_targets updates."""
+        user = getattr(self, "_user_targets", None)
+        if user is None:
             return getattr(self, name)
-        return dev[0 if name == "Sk" else 1].cpu().numpy()
+        return user[0 if name == "Sk" else 1].detach().cpu().numpy()
+
+    # --- split evaluation (the autograd function's forward / backward) -----
+
+    def energy_keep(self, x):
+        """Energy at x (a float64 device tensor) keeping what the gradient
+        needs: (token, E) with a nonzero token naming this evaluation, or
+        (0, None) for potentials without the split."""
+        e = ctypes.c_double()
+        token = ctypes.c_uint64()
+        self._last_x = x
+        _lib.check(
+            _lib.load().fresco_pot_energy_keep(
+                self._h, ctypes.c_void_p(x.data_ptr()), ctypes.byref(e), ctypes.byref(token), stream_handle()
+            )
+        )
+        return (int(token.value), float(e.value)) if token.value else (0, None)
+
+    def gradient_kept(self, token, grad=None):
+        """Gradient of the kept evaluation ``token`` into grad (a float64
+        device tensor of ndof entries; None: only check that it is kept).
+        False when the potential no longer keeps that evaluation."""
+        ok = ctypes.c_int()
+        _lib.check(
+            _lib.load().fresco_pot_gradient_kept(
+                self._h, ctypes.c_uint64(token), ctypes.c_void_p(None if grad is None else grad.data_ptr()),
+                ctypes.byref(ok), stream_handle(),
+            )
+        )
+        return bool(ok.value)
 
     def target_grads(self):
         """(d phi/d S0, d phi/d V) at the last evaluation, float64 device

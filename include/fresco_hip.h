@@ -202,6 +202,25 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad,
                                double* energy, void* stream);
 
 /*
+ * The energy of fresco_pot_energy_gradient at x (device, float64), keeping the
+ * state its gradient needs (for NUwU: the sorted points and the pullback
+ * spectrum) so that fresco_pot_gradient_kept can finish the pass later: the
+ * forward and backward halves of a differentiable loss.  *token (nonzero)
+ * names this evaluation; *token = 0 (energy untouched) when the potential has
+ * no such split.  No reference counterpart (the reference evaluates energy and
+ * gradient together, nuwu.hpp:210-317).
+ */
+int fresco_pot_energy_keep(fresco_pot* pot, const double* x, double* energy, uint64_t* token, void* stream);
+
+/*
+ * The gradient (device, float64, ndof) of the kept evaluation named token;
+ * *ok = 0 (grad untouched) when the potential no longer keeps it (another
+ * evaluation or a target update since).  grad may be NULL: only the check.
+ * The coordinates passed to fresco_pot_energy_keep must still hold their values.
+ */
+int fresco_pot_gradient_kept(fresco_pot* pot, uint64_t token, double* grad, int* ok, void* stream);
+
+/*
  * Copy the last evaluation's rho_k (complex, interleaved re/im, reference flat mode
  * order, length = number of modes) into rho[2*m] (device, float64).  The S(k) the
  * reference would compute from it is |rho_k|^2 / N (nuwu.hpp:262-264).
@@ -230,7 +249,8 @@ int fresco_pot_backward(fresco_pot* pot, double* dS0, double* dV, void* stream);
  * No reference counterpart (constructor constants there).  For NUwU a mode
  * switched on outside the constructor's active set (V_k = V_-k = 0 before)
  * rebuilds the mode-pair tables on the host; otherwise the update stays on
- * the device.  Synchronises the stream once.
+ * the device.  Synchronises the stream once.  V may be NULL: the weights stay
+ * and only S0 changes (no synchronisation).
  */
 int fresco_pot_set_targets(fresco_pot* pot, const double* Sk, const double* V, void* stream);
 

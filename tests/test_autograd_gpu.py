@@ -244,3 +244,105 @@ def test_set_targets_switches_on_new_modes(mods):
 
     E2 = pickle.loads(pickle.dumps(pot)).get_energy(p)
     assert abs(E2 - E0) / E0 < 1e-9
+
+
+def _small_case(prec, seed=31):
+    rng = np.random.default_rng(seed)
+    d, n, N = 2, 17, 300
+    M = n ** d
+    radii = rng.uniform(0.6, 1.4, N)
+    Sk = rng.uniform(0.2, 2, M)
+    Sk[::3] = 0
+    V = rng.uniform(0, 1, M)
+    V[::7] = 0
+    L = np.array([1.0, 1.5])
+    return radii, Sk, V, L, rng, N, d
+
+
+@pytest.mark.parametrize("prec,eps", [("fp64-adjoint", 1e-12), ("fp32", 1e-6), ("fp64", 1e-12)])
+def test_split_evaluation_equals_fused_pass(mods, prec, eps):
+    """energy_keep (spreading, transform, loss, pullback) followed by
+    gradient_kept (inverse transform, interpolation) gives bitwise the
+    energy and gradient of the fused pass; any other evaluation or a target
+    update drops the kept state."""
+    A, P = mods
+    radii, Sk, V, L, rng, N, d = _small_case(prec)
+    pot = P.NUwU(radii, 0, Sk, V, L, eps, precision=prec)
+    x = torch.tensor(rng.uniform(0, 1, N * d), device="cuda")
+    E0, g0 = pot.get_energy_gradient(x)
+    tok, E1 = pot.energy_keep(x)
+    assert tok and E1 == E0
+    g1 = torch.empty_like(x)
+    assert pot.gradient_kept(tok, g1) and torch.equal(g1, g0)
+    assert pot.gradient_kept(tok)
+    pot.get_energy(x)
+    assert not pot.gradient_kept(tok)
+    tok2, _ = pot.energy_keep(x)
+    assert tok2 != tok and not pot.gradient_kept(tok) and pot.gradient_kept(tok2)
+    pot.set_targets(Sk, V)
+    assert not pot.gradient_kept(tok2)
+
+
+def test_autograd_lazy_backward_and_interleaved_evaluations(mods):
+    """Forward keeps the pass half done; backward finishes it.  With another
+    evaluation of the same potential, or a target update, between the two,
+    backward evaluates again at the forward's coordinates and targets (and
+    leaves the potential's current targets in place).  Gradients against
+    the oracle each time."""
+    A, P = mods
+    radii, Sk, V, L, rng, N, d = _small_case("fp64-adjoint", seed=41)
+    pot = P.NUwU(radii, 0, Sk, V, L, 1e-12, precision="fp64-adjoint")
+    scale = np.tile(TWO_PI / L, N)
+
+    def oracle(p, s, v):
+        ref = O.NUwU(radii, 0, s, v, L)
+        E0, g0 = ref.get_energy_gradient(p)
+        dS0, dV = O.target_grads(ref.rho(p), s, v, N)
+        return E0, g0 * scale, dS0, dV
+
+    p1, p2 = rng.uniform(0, 1, N * d), rng.uniform(0, 1, N * d)
+    Sk2 = Sk * rng.uniform(0.5, 1.5, Sk.size)
+    # 1: plain forward / backward
+    x1 = torch.tensor(p1, device="cuda", requires_grad=True)
+    S0 = torch.tensor(Sk, requires_grad=True)
+    Vt = torch.tensor(V, requires_grad=True)
+    E = A.structure_factor_loss(pot, x1, S0, Vt)
+    E.backward()
+    E0, gx0, dS0, dV = oracle(p1, Sk, V)
+    assert abs(E.item() - E0) / abs(E0) < 1e-9
+    assert rel(x1.grad.cpu(), gx0) < 1e-9 and rel(S0.grad, dS0) < 1e-9 and rel(Vt.grad, dV) < 1e-9
+    # 2: two forwards, backward of the first after the second (re-evaluated)
+    xa = torch.tensor(p1, device="cuda", requires_grad=True)
+    xb = torch.tensor(p2, device="cuda", requires_grad=True)
+    Ea = A.structure_factor_loss(pot, xa, S0, Vt)
+    Eb = A.structure_factor_loss(pot, xb, S0, Vt)
+    Ea.backward()
+    Eb.backward()
+    assert rel(xa.grad.cpu(), gx0) < 1e-9
+    assert rel(xb.grad.cpu(), oracle(p2, Sk, V)[1]) < 1e-9
+    # 3: a target update between forward and backward
+    xc = torch.tensor(p1, device="cuda", requires_grad=True)
+    S0c = torch.tensor(Sk, requires_grad=True)
+    Ec = A.structure_factor_loss(pot, xc, S0c, Vt)
+    pot.set_targets(Sk2, V)
+    Ec.backward()
+    assert rel(xc.grad.cpu(), gx0) < 1e-9 and rel(S0c.grad, dS0) < 1e-9
+    E2 = oracle(p1, Sk2, V)[0]
+    assert abs(pot.get_energy(p1) - E2) / abs(E2) < 1e-9  # the update is still in place
+    # 4: the same target tensors again, then S0 changed in place (V kept)
+    S0d = torch.tensor(Sk, requires_grad=True)
+    for it in range(3):
+        xd = torch.tensor(p1, device="cuda", requires_grad=True)
+        Ed = A.structure_factor_loss(pot, xd, S0d, Vt)
+        Ed.backward()
+        s_now = S0d.detach().numpy().copy()
+        Eo, go, dSo, _ = oracle(p1, s_now, V)
+        assert abs(Ed.item() - Eo) / abs(Eo) < 1e-9
+        assert rel(xd.grad.cpu(), go) < 1e-9 and rel(S0d.grad, dSo) < 1e-9
+        S0d.grad = None
+        with torch.no_grad():
+            S0d.mul_(1.1)
+    # 5: forward without backward, then a plain evaluation: nothing left over
+    Ef = A.structure_factor_loss(pot, torch.tensor(p2, device="cuda", requires_grad=True), S0, Vt)
+    assert abs(Ef.item() - oracle(p2, Sk, V)[0]) / abs(Ef.item()) < 1e-9
+    assert abs(pot.get_energy(p1) - E0) / abs(E0) < 1e-9
