@@ -1006,6 +1006,11 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     halo_tab.upload(tabs.data(), tabs.size());
     const int64_t wg = (int64_t)g.nbins * g.batch;
     halo2_parts = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + wg - 1) / wg));
+    // FP32 strengths on 16 x 16 bins: the matrix-core spreader
+    // (FRESCO_SPREAD2=halo: the LDS one)
+    const char* env2 = std::getenv("FRESCO_SPREAD2");
+    halo2_mfma = sizeof(T) == 4 && g.tile[0] == 16 && g.tile[1] == 16 && halo2_hs == 24 &&
+                 !(env2 && std::string(env2) == "halo");
     halo_buf.allocate((size_t)wg * halo2_parts * (size_t)halo2_hs * halo2_rows);
   }
   if (halo_ok) {
@@ -2386,6 +2391,189 @@ __global__ void __launch_bounds__(64)
   for (int e = lane; e < helems; e += 64) out[e] = tile[e];
 }
 
+// The 2D halo spreading of FP32 strengths on the matrix cores (16 x 16 bins,
+// w <= 8), one wavefront per (bin, pattern, slice).  The 23 x 23 halo tile
+// is covered by four 16 x 16 windows [8 hx, 8 hx + 16) x [8 hy, 8 hy + 16)
+// (hx, hy = 0, 1: the bin's halves the first cell lies in); a window is
+//   D[y][x] = sum_p  wy_p[y - dy_p]  *  c_p wx_p[x - dx_p]
+// (dx, dy = the first cell's offset in its window, kernel values zero
+// outside [0, 8)): one v_mfma_f32_16x16x4f32 per 4 points, the four windows
+// held in accumulator registers.  The slice's points are ranked by window
+// (stable ballot ranks, runs padded to whole chunks of 8 slots with empty
+// slots of weight 0), the kernel values of a chunk evaluated in LDS records
+// while the previous chunk's products run.  At the end the windows are added
+// into an LDS tile in a fixed order and the tile is written out: the sum
+// order is fixed, the result bitwise reproducible.
+constexpr int H2_WIN = 384;  // points ranked (and staged in LDS) per window of a slice
+constexpr int H2_CH = 8;     // slots per chunk (one key, two products)
+constexpr int H2_RS = 48;    // floats per slot record: ay[16], bx[16]
+constexpr int H2_NK = 4;     // keys: hx + 2 hy
+constexpr int H2_HS = 24;    // halo row stride of 16 x 16 bins
+
+__global__ void __launch_bounds__(64)
+    k_spread2_halo_mfma(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
+                        const int* __restrict__ bin_start, SpreadDev<float> a, int hrows, float* __restrict__ halo) {
+  constexpr int MAXSLOT = H2_WIN + (H2_CH - 1) * H2_NK;
+  constexpr int WPL = H2_WIN / 64;
+  __shared__ __align__(16) float recs[2][H2_CH * H2_RS];
+  __shared__ __align__(16) float4 wrec[H2_WIN];
+  __shared__ uint32_t wsc[H2_WIN];
+  __shared__ uint16_t perm[MAXSLOT];
+  __shared__ int kend[H2_NK];
+  __shared__ int run[H2_NK];
+  __shared__ float tile[24 * H2_HS];
+  const int lane = threadIdx.x;
+  const int helems = H2_HS * hrows;
+  const int nbins = a.ntile[0] * a.ntile[1];
+  bin_start += (int64_t)blockIdx.y * nbins;
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.z) / gridDim.z);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.z + 1)) / gridDim.z);
+  float* out = halo + (((int64_t)blockIdx.y * nbins + b) * gridDim.z + blockIdx.z) * helems;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  const int t0[2] = {(b % a.ntile[0]) * a.tile[0], (b / a.ntile[0]) * a.tile[1]};
+  auto rel = [&](uint32_t sc, int q) {
+    int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+    if (u < 0) u += a.nf[q];
+    return u;
+  };
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int kj = lane & 7;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  hm_f4 acc[H2_NK];
+#pragma unroll
+  for (int k = 0; k < H2_NK; ++k) acc[k] = hm_f4{0.f, 0.f, 0.f, 0.f};
+  for (int ws = beg; ws < end; ws += H2_WIN) {
+    const int wn = min(H2_WIN, end - ws);
+    int wkey[WPL];
+    {
+      float4 wr[WPL];
+#pragma unroll
+      for (int u = 0; u < WPL; ++u) {
+        const int e = u * 64 + lane;
+        wkey[u] = e < wn ? (int)scell[ws + e] : 0;
+        wr[u] = e < wn ? rec[ws + e] : float4{0.f, 0.f, 0.f, 0.f};
+      }
+      __syncthreads();  // the previous window's chunks are done with wrec / wsc
+#pragma unroll
+      for (int u = 0; u < WPL; ++u) {
+        wrec[u * 64 + lane] = wr[u];
+        wsc[u * 64 + lane] = (uint32_t)wkey[u];
+      }
+    }
+    if (lane < H2_NK) run[lane] = 0;
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      const uint32_t sc = (uint32_t)wkey[u];
+      wkey[u] = u * 64 + lane < wn ? ((rel(sc, 0) >> 3) & 1) + 2 * ((rel(sc, 1) >> 3) & 1) : -1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WPL; ++u)
+      if (wkey[u] >= 0) atomicAdd(&run[wkey[u]], 1);
+    __syncthreads();
+    const int cnt = lane < H2_NK ? run[lane] : 0;
+    const int pad = (cnt + H2_CH - 1) / H2_CH * H2_CH;
+    int inc = pad;
+#pragma unroll
+    for (int off = 1; off < H2_NK; off <<= 1) {
+      const int u = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += u;
+    }
+    const int nslot = __builtin_amdgcn_readfirstlane(__shfl(inc, H2_NK - 1, 64));
+    const int st = inc - pad;
+    __syncthreads();
+    if (lane < H2_NK) {
+      run[lane] = st;
+      kend[lane] = st + pad;
+    }
+    for (int q = lane; q < nslot; q += 64) perm[q] = 0xFFFF;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      if (u * 64 >= wn) break;
+      const int key = wkey[u];
+      const bool valid = key >= 0;
+      uint64_t same = __ballot(valid);
+#pragma unroll
+      for (int bt = 0; bt < 2; ++bt) {
+        const bool bit = (key >> bt) & 1;
+        const uint64_t mk = __ballot(bit);
+        same &= bit ? mk : ~mk;
+      }
+      const int rank = __popcll(same & lt);
+      if (valid) perm[run[key] + rank] = (uint16_t)(u * 64 + lane);
+      __syncthreads();
+      if (valid && rank == 0) run[key] += __popcll(same);
+      __syncthreads();
+    }
+    // branch-free chunk preparation, as in k_spread3_halo_mfma
+    uint32_t scn;
+    float4 rcn;
+    auto load = [&](int c0) {
+      const int q = c0 + (lane >> 3);
+      const int praw = (int)perm[min(q, MAXSLOT - 1)];
+      const bool empty = q >= nslot || praw >= wn;
+      const int pq = empty ? 0 : praw;
+      const uint32_t sc = wsc[pq];
+      rcn = wrec[pq];
+      scn = empty ? 0xFFFFFFFFu : sc;
+    };
+    auto values = [&](float* rb) {
+      float* rr = rb + (lane >> 3) * H2_RS;
+      const float4 r = rcn;
+      const bool empty = scn == 0xFFFFFFFFu;
+      const int dx = rel(scn, 0) & 7;
+      const int dy = rel(scn, 1) & 7;
+      const bool on = kj < a.w;
+      float wx = es_phi<float>((float(kj) - r.x) * a.k.two_over_w, a.k);
+      float wy = es_phi<float>((float(kj) - r.y) * a.k.two_over_w, a.k);
+      wx = on && !empty ? r.w * wx : 0.f;
+      wy = on ? wy : 0.f;
+      rr[kj + dy] = wy;
+      rr[(kj + dy + 8) & 15] = 0.f;
+      rr[16 + kj + dx] = wx;
+      rr[16 + ((kj + dx + 8) & 15)] = 0.f;
+    };
+    int kb[H2_NK + 1];
+    kb[0] = 0;
+#pragma unroll
+    for (int K = 0; K < H2_NK; ++K) kb[K + 1] = __builtin_amdgcn_readfirstlane(kend[K]);
+    load(0);
+    values(recs[0]);
+    load(H2_CH);
+    int c0 = 0, cb = 0;
+    hm_static_for<0, H2_NK>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      for (; c0 < kb[K + 1]; c0 += H2_CH, cb ^= 1) {
+        const float* r0 = recs[cb] + k4 * H2_RS;
+        const float* r1 = r0 + 4 * H2_RS;
+        const float av0 = r0[i16], bx0 = r0[16 + i16];
+        const float av1 = r1[i16], bx1 = r1[16 + i16];
+        values(recs[cb ^ 1]);
+        load(c0 + 2 * H2_CH);
+        acc[K] = __builtin_amdgcn_mfma_f32_16x16x4f32(av0, bx0, acc[K], 0, 0, 0);
+        acc[K] = __builtin_amdgcn_mfma_f32_16x16x4f32(av1, bx1, acc[K], 0, 0, 0);
+      }
+    });
+    __syncthreads();
+  }
+  // the four windows into the tile, in key order: register r of lane
+  // (k4, i16) holds row 8 hy + 4 k4 + r, column 8 hx + i16
+  for (int e = lane; e < 24 * H2_HS; e += 64) tile[e] = 0.f;
+  __syncthreads();
+#pragma unroll
+  for (int K = 0; K < H2_NK; ++K) {
+    const int hx = K & 1, hy = K >> 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tile[(8 * hy + 4 * k4 + r) * H2_HS + 8 * hx + i16] += acc[K][r];
+    __syncthreads();
+  }
+  for (int e = lane; e < helems; e += 64) out[e] = tile[e];
+}
+
 // per fine cell of every pattern: the sum of the halo tiles covering it
 // (tabs as k_merge_halo's, x then y), slices in order
 template <typename T>
@@ -2439,9 +2627,18 @@ static bool launch_spread2_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   const int helems = hs * hrows;
   {
     ProfScope ps(ST_SPREAD, s);
-    hipLaunchKernelGGL(k_spread2_halo<T>, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64),
-                       sizeof(T) * (size_t)helems, s, p.scell.get(), p.rec.get(), p.bin_start.get(), a, hs, hrows,
-                       sp.halo_buf.get());
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (sp.halo2_mfma) {
+        hipLaunchKernelGGL(k_spread2_halo_mfma, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64), 0, s,
+                           p.scell.get(), p.rec.get(), p.bin_start.get(), a, hrows, sp.halo_buf.get());
+        done = true;
+      }
+    }
+    if (!done)
+      hipLaunchKernelGGL(k_spread2_halo<T>, dim3(g.nbins, g.batch, sp.halo2_parts), dim3(64),
+                         sizeof(T) * (size_t)helems, s, p.scell.get(), p.rec.get(), p.bin_start.get(), a, hs, hrows,
+                         sp.halo_buf.get());
     FRESCO_HIP_CHECK(hipGetLastError());
   }
   ProfScope pm(ST_MERGE, s);

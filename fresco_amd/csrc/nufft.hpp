@@ -159,6 +159,7 @@ class Spreader {
   bool halo2_ok = false;
   int halo2_hs = 0, halo2_rows = 0;  // halo tile row stride and rows
   int halo2_parts = 1;               // point-range slices per bin
+  bool halo2_mfma = false;           // FP32 16 x 16 bins: matrix-core spreader
 };
 
 // Interpolation of a real fine grid at the sorted points.

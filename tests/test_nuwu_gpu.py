@@ -302,6 +302,43 @@ def test_matrix_core_spreader(NUwU, case, monkeypatch):
         assert rel(res[mode][1], res["halo4:2"][1]) < 2e-6
 
 
+@pytest.mark.parametrize("case", ["uniform", "clustered"])
+def test_matrix_core_spreader_2d(NUwU, case, monkeypatch):
+    """2D FP32 halo spreading as matrix products (16 x 16 bins: four 16 x 16
+    windows of the 23 x 23 halo tile, one product per 4 points of a window,
+    windows summed into the tile in a fixed order): parity with the exact
+    sums, agreement with the LDS spreader to FP32 rounding, bitwise repeats.
+    Clustered points put hundreds to thousands of points into single bins
+    (several ranking windows per slice)."""
+    n, N = 65, 8000
+    radii, Sk, V, p = make_case(2, n, N, seed=91)
+    if case == "clustered":
+        rng = np.random.default_rng(92)
+        q = rng.uniform(0, 1, (N, 2))
+        q[: int(0.7 * N)] = 0.4 + 0.03 * rng.uniform(0, 1, (int(0.7 * N), 2))
+        p = q.ravel()
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(2))
+    E0, g0 = ref.get_energy_gradient(p)
+    res = {}
+    for mode in ("mfma", "halo"):
+        if mode == "halo":
+            monkeypatch.setenv("FRESCO_SPREAD2", "halo")
+        else:
+            monkeypatch.delenv("FRESCO_SPREAD2", raising=False)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(2), 1e-6, precision="fp32")
+        info = pot.plan_info()
+        print(case, mode, info, flush=True)
+        assert info["tile"][:2] == [16, 16]
+        E, g = pot.get_energy_gradient(p)
+        assert abs(E - E0) / abs(E0) < 1e-5
+        assert rel(g, g0) < 1e-5
+        E2, g2 = pot.get_energy_gradient(p)
+        assert E == E2 and np.array_equal(g, g2)
+        res[mode] = (E, g)
+    assert abs(res["mfma"][0] - res["halo"][0]) / abs(E0) < 2e-6
+    assert rel(res["mfma"][1], res["halo"][1]) < 2e-6
+
+
 @pytest.mark.parametrize("d,n,N", [(3, 33, 5000), (3, 21, 2047), (2, 65, 4097), (1, 129, 3000)])
 def test_direct_binning_equals_radix_sort(NUwU, d, n, N, monkeypatch):
     """Plans whose bins fit one 11-bit digit are binned by one counting pass
