@@ -3342,13 +3342,32 @@ __global__ void __launch_bounds__(G8_THREADS)
   if (beg == end) return;  // uniform across the workgroup
   const int t0x = (b % a.ntile[0]) * a.tile[0];
   const int t0y = (b / a.ntile[0]) * a.tile[1];
-  const int tot = a.rx * a.ry;
-  for (int e = tid; e < tot; e += G8_THREADS) {
-    const int lx = e % a.rx;
-    const int ly = e / a.rx;
-    const int gx = (t0x + lx) % a.nf[0];  // the region may wrap
-    const int gy = (t0y + ly) % a.nf[1];
-    reg[ly * a.srx + lx] = grid[(int64_t)gy * a.nf[0] + gx];
+  // region rows of up to 32 cells, thread (row slot, x); every load of a
+  // thread is issued before its LDS stores (one memory latency per pass of
+  // four rows per thread instead of one per cell); the region may wrap
+  {
+    const int lx = tid & 31;
+    int gx = t0x + lx;
+    while (gx >= a.nf[0]) gx -= a.nf[0];
+    constexpr int RPP = G8_THREADS / 32;  // rows per pass
+    for (int r0 = 0; r0 < a.ry; r0 += 4 * RPP) {
+      T v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ly = r0 + q * RPP + (tid >> 5);
+        v[q] = T(0);
+        if (ly < a.ry && lx < a.rx) {
+          int gy = t0y + ly;
+          while (gy >= a.nf[1]) gy -= a.nf[1];
+          v[q] = grid[(int64_t)gy * a.nf[0] + gx];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ly = r0 + q * RPP + (tid >> 5);
+        if (ly < a.ry && lx < a.rx) reg[ly * a.srx + lx] = v[q];
+      }
+    }
   }
   __syncthreads();
   const int ja = tid & 7;
@@ -3419,6 +3438,113 @@ __global__ void __launch_bounds__(G8_THREADS)
   }
 }
 
+// 2D interpolation of FP32 grids (w <= 8), one lane per point: the lane
+// evaluates the point's 8 x and 8 y kernel values (and derivatives) itself
+// and contracts its 8 x 8 cells from the staged region, x first (rows
+// T[y] = sum_x g wx, and with dwx), then y; no cross-lane traffic.  In 2D the
+// contraction is short (64 cells), so the eight-lane layout's shared
+// broadcasts and reductions cost more than the redundancy they save.
+template <bool DERIV>
+__global__ void __launch_bounds__(256)
+    k_interp2_lane1(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
+                    const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const float* __restrict__ grid,
+                    InterpDev<float> a, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* reg = reinterpret_cast<float*>(smem_raw);
+  const int tid = threadIdx.x;
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  bin_start += (int64_t)blockIdx.y * a.ntile[0] * a.ntile[1];
+  grid += (int64_t)blockIdx.y * a.nf[0] * a.nf[1];
+  const int b = blockIdx.x;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * blockIdx.z) / gridDim.z);
+  const int end = beg0 + (int)(((int64_t)len * (blockIdx.z + 1)) / gridDim.z);
+  if (beg == end) return;  // uniform across the workgroup
+  const int t0x = (b % a.ntile[0]) * a.tile[0];
+  const int t0y = (b / a.ntile[0]) * a.tile[1];
+  {
+    const int lx = tid & 31;
+    int gx = t0x + lx;
+    while (gx >= a.nf[0]) gx -= a.nf[0];
+    for (int r0 = 0; r0 < a.ry; r0 += 32) {
+      float v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ly = r0 + q * 8 + (tid >> 5);
+        v[q] = 0.f;
+        if (ly < a.ry && lx < a.rx) {
+          int gy = t0y + ly;
+          while (gy >= a.nf[1]) gy -= a.nf[1];
+          v[q] = grid[(int64_t)gy * a.nf[0] + gx];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ly = r0 + q * 8 + (tid >> 5);
+        if (ly < a.ry && lx < a.rx) reg[ly * a.srx + lx] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+  const bool on[8] = {0 < a.w, 1 < a.w, 2 < a.w, 3 < a.w, 4 < a.w, 5 < a.w, 6 < a.w, 7 < a.w};
+  for (int k = beg + tid; k < end; k += 256) {
+    const uint32_t sc = scell[k];
+    const float4 r = rec[k];
+    const uint32_t jout = idx[k];
+    int ux = (int)(sc & cmask) - t0x;
+    if (ux < 0) ux += a.nf[0];
+    int uy = (int)((sc >> a.cbits) & cmask) - t0y;
+    if (uy < 0) uy += a.nf[1];
+    float wx[8], dwx[8], wy[8], dwy[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      wx[j] = dwx[j] = wy[j] = dwy[j] = 0.f;
+      if (DERIV) {
+        float v0, d0, v1, d1;
+        es_phi_dphi<float>((float(j) - r.x) * a.k.two_over_w, a.k, v0, d0);
+        es_phi_dphi<float>((float(j) - r.y) * a.k.two_over_w, a.k, v1, d1);
+        wx[j] = on[j] ? v0 : 0.f;
+        dwx[j] = on[j] ? d0 : 0.f;
+        wy[j] = on[j] ? v1 : 0.f;
+        dwy[j] = on[j] ? d1 : 0.f;
+      } else {
+        const float v0 = es_phi<float>((float(j) - r.x) * a.k.two_over_w, a.k);
+        const float v1 = es_phi<float>((float(j) - r.y) * a.k.two_over_w, a.k);
+        wx[j] = on[j] ? v0 : 0.f;
+        wy[j] = on[j] ? v1 : 0.f;
+      }
+    }
+    const float* base = reg + uy * a.srx + ux;
+    float vx = 0.f, vy = 0.f;
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+      const float* row = base + y * a.srx;
+      float t1 = 0.f, t2 = 0.f;
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        const float g = row[x];
+        t1 += g * wx[x];
+        if (DERIV) t2 += g * dwx[x];
+      }
+      if (DERIV) {
+        vx += wy[y] * t2;
+        vy += dwy[y] * t1;
+      } else {
+        vx += wy[y] * t1;
+      }
+    }
+    const int64_t j = jout;
+    const double cw = (double)r.w;
+    if (DERIV) {
+      out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
+      out[j * a.ostride + 1] = cw * a.dscale[1] * (double)vy;
+    } else {
+      out[j * a.ostride + a.ocomp] = cw * (double)vx;
+    }
+  }
+}
+
 template <typename T, bool DERIV>
 static bool launch_interp2_g8(const SortedPoints<T>& p, const T* grid, const double* dscale, double* out,
                               int ostride, int ocomp, hipStream_t s) {
@@ -3442,9 +3568,19 @@ static bool launch_interp2_g8(const SortedPoints<T>& p, const T* grid, const dou
   a.k.bexp = sizeof(T) == 4 ? (T)(p.es.beta * M_LOG2E) : (T)p.es.beta;
   a.k.two_over_w = (T)(2.0 / p.es.w);
   const size_t lds = sizeof(T) * (size_t)a.rx * a.ry;
-  if (lds > 64 * 1024) return false;
+  if (lds > 64 * 1024 || a.rx > 32) return false;
   const int64_t wg = (int64_t)g.nbins * g.batch;
   const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(8, (2048 + wg - 1) / wg));
+  if constexpr (sizeof(T) == 4) {
+    // FRESCO_INTERP2=g8: the eight-lane kernel instead
+    const char* env = std::getenv("FRESCO_INTERP2");
+    if (!(env && std::string(env) == "g8")) {
+      hipLaunchKernelGGL((k_interp2_lane1<DERIV>), dim3(g.nbins, g.batch, chunks), dim3(256), lds, s,
+                         p.scell.get(), p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+      FRESCO_HIP_CHECK(hipGetLastError());
+      return true;
+    }
+  }
   hipLaunchKernelGGL((k_interp2_g8<T, DERIV>), dim3(g.nbins, g.batch, chunks), dim3(G8_THREADS), lds, s,
                      p.scell.get(), p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
   FRESCO_HIP_CHECK(hipGetLastError());
