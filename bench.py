@@ -247,8 +247,7 @@ def main():
             "nf": nf,
             "w": info32["w"],
             "tile": info32["tile"],
-            "spreader": "%s, %d slices/bin (%s)" % (info32["spreader"], info32["spread_parts"],
-                                                     info32["spread_source"]),
+            "spreader": spreader_label(info32),
         },
         "points_per_s": round(ev32 * N, 1),
         "fp64_plain": {
@@ -280,6 +279,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def spreader_label(info):
+    """The spreader in use, its halo tile parts per bin and where the pick came from."""
+    name, parts = info["spreader"], info["spread_parts"]
+    if name == "halom":
+        split = "%d slice(s) x 2 x windows" % (parts // 2)
+    else:
+        split = "%d slice(s)" % parts
+    return "%s:%d, %s (%s)" % (name, parts, split, info["spread_source"])
 
 
 def measured_traffic(stage, info):

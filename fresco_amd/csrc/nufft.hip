@@ -1042,9 +1042,10 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
-    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] | halom[:<slices>] pins the
+    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] | halom[:<parts>] pins the
     // choice (reproducible rounding across processes) instead of timing the
-    // candidates; halom (FP32 strengths, 16 x 8 x 8 bins) falls back to halo4 otherwise
+    // candidates; halom (FP32 strengths, 16 x 8 x 8 bins; parts = 2 x windows
+    // per slice) falls back to halo4 otherwise
     if (const char* env = std::getenv("FRESCO_SPREAD3")) {
       const std::string v(env);
       if (v == "reg") {
