@@ -178,7 +178,7 @@ def main():
     # headline run
     dom_ms = live[0] / live[1]
     achieved = alg_bytes[dom] / (dom_ms * 1e-3) / 1e9
-    kernels = {"halom": "k_spread3_halo_mfma", "halo4": "k_spread3_halo_mw", "halo": "k_spread3_halo",
+    kernels = {"halom": "k_spread3_halo_mfma2", "halo4": "k_spread3_halo_mw", "halo": "k_spread3_halo",
                "tiles": "k_spread3_reg"}
     kname = kernels.get(info32["spreader"], "k_spread") if dom == "spread" else "k_interp3_g8"
     w = info32["w"]
@@ -285,7 +285,7 @@ def spreader_label(info):
     """The spreader in use, its halo tile parts per bin and where the pick came from."""
     name, parts = info["spreader"], info["spread_parts"]
     if name == "halom":
-        split = "%d slice(s) x 2 x windows" % (parts // 2)
+        split = "%d slice(s) x 2 x windows, one halo tile per slice" % (parts // 2)
     else:
         split = "%d slice(s)" % parts
     return "%s:%d, %s (%s)" % (name, parts, split, info["spread_source"])

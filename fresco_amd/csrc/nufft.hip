@@ -1988,9 +1988,10 @@ __global__ void __launch_bounds__(64 * NW)
 }
 
 // The halo spreading of FP32 strengths on the matrix cores (16 x 8 x 8
-// bins), one wavefront per bin slice and x window [8 h, 8 h + 16) (h = 0, 1:
-// the points whose first x cell lies in the bin's left or right 8 columns),
-// the window's 15 planes held in accumulator registers.  Plane z is
+// bins).  A bin slice's points split by the x cell their support starts in
+// into two windows of the halo tile, columns [8 h, 8 h + 16) (h = 0, 1: the
+// bin's left or right 8 columns), each the work of one wavefront that holds
+// the window's 15 planes in accumulator registers.  Plane z of a window is
 //   D_z[y][j] = sum_p  wy_p[y - uy_p]  *  wz_p[z - uz_p] c_p wx_p[8 h + j - ux_p]
 // with the kernel values zero outside [0, 8): a 16 x 16 product over the
 // points (K, 4 per v_mfma_f32_16x16x4f32), A = the points' y values placed at
@@ -1998,12 +1999,9 @@ __global__ void __launch_bounds__(64 * NW)
 // cell.  The window's points are ranked by the z of their first cell, each
 // run padded to whole chunks of 8 slots (empty slots weigh 0), so the 4
 // points of a product share it and add to the 8 planes uz .. uz + 7: 8
-// products per 4 points and no read-modify-write of a tile at all.  Each
-// wavefront writes its window's columns of a halo tile (the other columns
-// stay zero), and the merge sums the two windows with the slices.  Ranking
-// is stable (ballot ranks in index order), so the summation order is fixed
-// and the result bitwise reproducible.
-constexpr int HM_WIN = 384;   // points ranked (and staged in LDS) per window of a slice
+// products per 4 points and no read-modify-write of a tile while points are
+// added.  Ranking is stable (ballot ranks in index order), so the summation
+// order is fixed and the result bitwise reproducible.
 constexpr int HM_CH = 8;      // slots per chunk (one key, two products; runs padded to it)
 constexpr int HM_RS = 48;     // floats per slot record: ay[16], bx[16], wz[8]
 constexpr int HM_NK = 8;      // keys: uz
@@ -2028,36 +2026,52 @@ __device__ __forceinline__ void hm_products(hm_f4 (&acc)[HP_Z], float av, float 
     acc[UZ + m] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx * wz[m], acc[UZ + m], 0, 0, 0);
 }
 
-// grid (bins, 2 * slices): blockIdx.y = 2 slice + h; halo parts = gridDim.y
-__global__ void __launch_bounds__(64)
-    k_spread3_halo_mfma(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
-                        const int* __restrict__ bin_start, SpreadDev<float> a, float* __restrict__ halo) {
+// Both windows of a (bin, slice) run in one workgroup of two wavefronts
+// (wavefront h = window h) that share one staged copy of the slice's
+// records and first cells (every point is read from memory once), rank the
+// slice in rounds of up to WIN points (each window's z runs padded to whole
+// chunks once per round), read the operands of chunk c + 1 from LDS while
+// the products of chunk c run, and at the end add the two windows into one
+// halo tile in LDS (the right window's columns, then the left window's
+// added) that the workgroup writes whole: one tile per (bin, slice), which
+// the merge sums like the LDS spreaders' tiles.
+template <int WIN>
+__global__ void __launch_bounds__(128)
+    k_spread3_halo_mfma2(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
+                         const int* __restrict__ bin_start, SpreadDev<float> a, float* __restrict__ halo) {
   constexpr int HS_Y = HaloShape<8>::HS_Y;
   constexpr int ELEMS = HaloShape<8>::ELEMS;
   static_assert(HS_Y == 16, "the 16 rows of a product are the tile's y rows");
-  constexpr int MAXSLOT = HM_WIN + (HM_CH - 1) * HM_NK;
-  constexpr int WPL = HM_WIN / 64;  // window points per lane
-  __shared__ __align__(16) float recs[2][HM_CH * HM_RS];  // kernel values of two chunks
-  __shared__ __align__(16) float4 wrec[HM_WIN];  // the window's records and first cells
-  __shared__ uint32_t wsc[HM_WIN];
-  __shared__ uint16_t perm[MAXSLOT];
-  __shared__ int kend[HM_NK];
-  __shared__ int run[HM_NK];
-  const int lane = threadIdx.x;
+  static_assert(WIN % 128 == 0, "a round is staged by 128 threads");
+  constexpr int MAXSLOT = WIN + (HM_CH - 1) * HM_NK;
+  constexpr int WPL = WIN / 64;   // round points per lane (ranking, per wavefront)
+  constexpr int SPL = WIN / 128;  // round points per thread (staging)
+  // the round's records and first cells, and each window's slot order;
+  // after the last round the same bytes hold the summed halo tile
+  constexpr int TILE = HP_Z * HS_Y * HS_X;
+  constexpr int STAGE = WIN * (16 + 4) + 2 * MAXSLOT * 2;
+  __shared__ __align__(16) unsigned char smem[STAGE > TILE * 4 ? STAGE : TILE * 4];
+  float4* wrec = reinterpret_cast<float4*>(smem);
+  uint32_t* wsc = reinterpret_cast<uint32_t*>(smem + WIN * 16);
+  uint16_t* perm = reinterpret_cast<uint16_t*>(smem + WIN * 20);
+  __shared__ __align__(16) float recs[2][2][HM_CH * HM_RS];  // [window][buffer]: kernel values of two chunks
+  __shared__ int kend[2][HM_NK];
+  __shared__ int run[2][HM_NK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int h = tid >> 6;
   const int b = blockIdx.x;
-  const int h = blockIdx.y & 1;
-  const int nsl = gridDim.y >> 1, sl = blockIdx.y >> 1;
+  const int nsl = gridDim.y, sl = blockIdx.y;
   const int beg0 = bin_start[b];
   const int len = bin_start[b + 1] - beg0;
   const int beg = beg0 + (int)(((int64_t)len * sl) / nsl);
   const int end = beg0 + (int)(((int64_t)len * (sl + 1)) / nsl);
-  float* out = halo + ((int64_t)b * gridDim.y + blockIdx.y) * ELEMS;
+  float* out = halo + ((int64_t)b * nsl + sl) * ELEMS;
   const uint32_t cmask = (1u << a.cbits) - 1u;
   int t0[3];
   t0[0] = (b % a.ntile[0]) * a.tile[0];
   t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
   t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
-  // first cell relative to the bin (u < 0 only across the periodic wrap)
   auto rel = [&](uint32_t sc, int q) {
     int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
     if (u < 0) u += a.nf[q];
@@ -2066,44 +2080,46 @@ __global__ void __launch_bounds__(64)
   const int i16 = lane & 15, k4 = lane >> 4;
   const int kj = lane & 7;
   const uint64_t lt = (1ull << lane) - 1ull;
+  uint16_t* wperm = perm + h * MAXSLOT;
+  int* wrun = run[h];
+  int* wkend = kend[h];
   hm_f4 acc[HP_Z];
 #pragma unroll
   for (int z = 0; z < HP_Z; ++z) acc[z] = hm_f4{0.f, 0.f, 0.f, 0.f};
-  for (int ws = beg; ws < end; ws += HM_WIN) {
-    const int wn = min(HM_WIN, end - ws);
-    // the window's records and first cells, loaded once (coalesced, all
-    // loads in flight together) and staged in LDS for the chunks
-    int wkey[WPL];
+  for (int ws = beg; ws < end; ws += WIN) {
+    const int wn = min(WIN, end - ws);
     {
-      float4 wr[WPL];
+      float4 wr[SPL];
+      uint32_t wk[SPL];
 #pragma unroll
-      for (int u = 0; u < WPL; ++u) {
-        const int e = u * 64 + lane;
-        wkey[u] = e < wn ? (int)scell[ws + e] : 0;
+      for (int u = 0; u < SPL; ++u) {
+        const int e = u * 128 + tid;
+        wk[u] = e < wn ? scell[ws + e] : 0u;
         wr[u] = e < wn ? rec[ws + e] : float4{0.f, 0.f, 0.f, 0.f};
       }
-      __syncthreads();  // the previous window's chunks are done with wrec / wsc
+      __syncthreads();  // both windows are done with the previous round's records
 #pragma unroll
-      for (int u = 0; u < WPL; ++u) {
-        wrec[u * 64 + lane] = wr[u];
-        wsc[u * 64 + lane] = (uint32_t)wkey[u];
+      for (int u = 0; u < SPL; ++u) {
+        wrec[u * 128 + tid] = wr[u];
+        wsc[u * 128 + tid] = wk[u];
       }
     }
-    if (lane < HM_NK) run[lane] = 0;
-    // key: z of the first cell, for the points of this x window; -1 others
+    if (lane < HM_NK) wrun[lane] = 0;
+    __syncthreads();
+    // key: z of the first cell, for the points of this wavefront's window; -1 others
+    int wkey[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
-      const uint32_t sc = (uint32_t)wkey[u];
-      const bool mine = u * 64 + lane < wn && ((rel(sc, 0) >> 3) & 1) == h;
+      const int e = u * 64 + lane;
+      const uint32_t sc = wsc[e];
+      const bool mine = e < wn && ((rel(sc, 0) >> 3) & 1) == h;
       wkey[u] = mine ? (rel(sc, 2) & 7) : -1;
     }
-    __syncthreads();
 #pragma unroll
     for (int u = 0; u < WPL; ++u)
-      if (wkey[u] >= 0) atomicAdd(&run[wkey[u]], 1);
+      if (wkey[u] >= 0) atomicAdd(&wrun[wkey[u]], 1);
     __syncthreads();
-    // runs padded to whole chunks: exclusive prefix over the keys
-    const int cnt = lane < HM_NK ? run[lane] : 0;
+    const int cnt = lane < HM_NK ? wrun[lane] : 0;
     const int pad = (cnt + HM_CH - 1) / HM_CH * HM_CH;
     int inc = pad;
 #pragma unroll
@@ -2115,12 +2131,13 @@ __global__ void __launch_bounds__(64)
     const int st = inc - pad;
     __syncthreads();
     if (lane < HM_NK) {
-      run[lane] = st;
-      kend[lane] = st + pad;
+      wrun[lane] = st;
+      wkend[lane] = st + pad;
     }
-    for (int q = lane; q < nslot; q += 64) perm[q] = 0xFFFF;
+    for (int q = lane; q < nslot; q += 64) wperm[q] = 0xFFFF;
     __syncthreads();
-    // stable placement, 64 points a round: rank among equal keys by ballots
+    // stable placement, 64 points a round per wavefront (wn is uniform, so
+    // both wavefronts run the same rounds and barriers)
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
       if (u * 64 >= wn) break;
@@ -2134,27 +2151,22 @@ __global__ void __launch_bounds__(64)
         same &= bit ? mk : ~mk;
       }
       const int rank = __popcll(same & lt);
-      if (valid) perm[run[key] + rank] = (uint16_t)(u * 64 + lane);
+      if (valid) wperm[min(wrun[key] + rank, MAXSLOT - 1)] = (uint16_t)(u * 64 + lane);
       __syncthreads();
-      if (valid && rank == 0) run[key] += __popcll(same);
+      if (valid && rank == 0) wrun[key] += __popcll(same);
       __syncthreads();
     }
-    // chunks of 8 slots, each of one key: the products of chunk c (plane
-    // offset known at compile time) run while the kernel values of chunk c + 1
-    // are evaluated into the other record buffer and the points of chunk
-    // c + 2 are read.  One wavefront: its LDS accesses complete in program
-    // order, so the buffers need no barrier.
+    // slot -> point: the permutation entry of chunk c + 3 is read one chunk
+    // before the record of that chunk, so neither waits on the other
     uint32_t scn;
     float4 rcn;
-    // lane (slot lane / 8, index kj); empty slots take point 0 of the window
-    // with weight 0
-    // (branch-free: slots at or past nslot read clamped addresses and come
-    // out empty, so a chunk's values and loads can be issued unconditionally
-    // between its products)
-    auto load = [&](int c0) {
-      const int q = c0 + (lane >> 3);
-      const int praw = (int)perm[min(q, MAXSLOT - 1)];
-      const bool empty = q >= nslot || praw >= wn;
+    int pslot, praw;
+    auto fetch = [&](int c0) {
+      pslot = c0 + (lane >> 3);
+      praw = (int)wperm[min(pslot, MAXSLOT - 1)];
+    };
+    auto load = [&]() {
+      const bool empty = pslot >= nslot || praw >= wn;
       const int pq = empty ? 0 : praw;
       const uint32_t sc = wsc[pq];
       rcn = wrec[pq];
@@ -2179,42 +2191,86 @@ __global__ void __launch_bounds__(64)
       rr[16 + ((kj + dx + 8) & 15)] = 0.f;
       rr[32 + kj] = wz;
     };
-    // key run bounds (slots), uniform
+    // the operands of one chunk: two products of 4 slots
+    struct Ops {
+      float av0, bx0, av1, bx1;
+      float4 zl0, zh0, zl1, zh1;
+    };
+    auto operands = [&](const float* rb) {
+      const float* r0 = rb + k4 * HM_RS;
+      const float* r1 = r0 + 4 * HM_RS;
+      Ops o;
+      o.av0 = r0[i16];
+      o.bx0 = r0[16 + i16];
+      o.zl0 = *reinterpret_cast<const float4*>(r0 + 32);
+      o.zh0 = *reinterpret_cast<const float4*>(r0 + 36);
+      o.av1 = r1[i16];
+      o.bx1 = r1[16 + i16];
+      o.zl1 = *reinterpret_cast<const float4*>(r1 + 32);
+      o.zh1 = *reinterpret_cast<const float4*>(r1 + 36);
+      return o;
+    };
     int kb[HM_NK + 1];
     kb[0] = 0;
 #pragma unroll
-    for (int K = 0; K < HM_NK; ++K) kb[K + 1] = __builtin_amdgcn_readfirstlane(kend[K]);
-    load(0);
-    values(recs[0]);
-    load(HM_CH);
+    for (int K = 0; K < HM_NK; ++K) kb[K + 1] = __builtin_amdgcn_readfirstlane(wkend[K]);
+    auto rbuf = [&](int i) { return &recs[0][0][0] + (2 * h + i) * (HM_CH * HM_RS); };
+    fetch(0);
+    load();
+    values(rbuf(0));
+    fetch(HM_CH);
+    load();
+    fetch(2 * HM_CH);
+    Ops cur = operands(rbuf(0));
     int c0 = 0, cb = 0;
+    // one wavefront per window: its LDS accesses complete in program order,
+    // so the chunk buffers need no barrier
     hm_static_for<0, HM_NK>([&](auto kc) {
       constexpr int K = decltype(kc)::value;
       for (; c0 < kb[K + 1]; c0 += HM_CH, cb ^= 1) {
-        const float* r0 = recs[cb] + k4 * HM_RS;
-        const float* r1 = r0 + 4 * HM_RS;
-        const float av0 = r0[i16], bx0 = r0[16 + i16];
-        const float4 zl0 = *reinterpret_cast<const float4*>(r0 + 32), zh0 = *reinterpret_cast<const float4*>(r0 + 36);
-        const float av1 = r1[i16], bx1 = r1[16 + i16];
-        const float4 zl1 = *reinterpret_cast<const float4*>(r1 + 32), zh1 = *reinterpret_cast<const float4*>(r1 + 36);
-        values(recs[cb ^ 1]);
-        load(c0 + 2 * HM_CH);
-        hm_products<K>(acc, av0, bx0, zl0, zh0);
-        hm_products<K>(acc, av1, bx1, zl1, zh1);
+        values(rbuf(cb ^ 1));
+        load();
+        fetch(c0 + 3 * HM_CH);
+        const Ops nxt = operands(rbuf(cb ^ 1));
+        hm_products<K>(acc, cur.av0, cur.bx0, cur.zl0, cur.zh0);
+        hm_products<K>(acc, cur.av1, cur.bx1, cur.zl1, cur.zh1);
+        cur = nxt;
       }
     });
     __syncthreads();
   }
-  // out: register r of lane (k4, i16) holds row y = 4 k4 + r, column
-  // x = 8 h + i16.  The tile's other 8 columns and its row 15 (beyond the
-  // 8 + 7 rows a bin's points reach) are zero and not written: the buffer
-  // holds zeros there from its last fill.
+  // register r of lane (k4, i16) holds row y = 4 k4 + r, column x = 8 h +
+  // i16 of every plane; rows 0..14 carry points (row 15 stays zero).  The
+  // tile [plane][row][24 columns] is the right window's columns 8..23 and the
+  // left window's 0..7, then the left window's 8..15 added (column 23 is 0).
+  float* tile = reinterpret_cast<float*>(smem);
+  constexpr int RS = 15 * HS_X;  // floats per plane in LDS
   const int xo = 8 * h + i16;
+  if (h == 1 || i16 < 8) {
 #pragma unroll
-  for (int z = 0; z < HP_Z; ++z)
+    for (int z = 0; z < HP_Z; ++z)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (r < 3 || k4 < 3) out[(z * HS_Y + 4 * k4 + r) * HS_X + xo] = acc[z][r];
+      for (int r = 0; r < 4; ++r)
+        if (r < 3 || k4 < 3) tile[z * RS + (4 * k4 + r) * HS_X + xo] = acc[z][r];
+  }
+  __syncthreads();
+  if (h == 0 && i16 >= 8) {
+#pragma unroll
+    for (int z = 0; z < HP_Z; ++z)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < 3 || k4 < 3) tile[z * RS + (4 * k4 + r) * HS_X + xo] += acc[z][r];
+  }
+  __syncthreads();
+  // rows 0..14 of each plane to the halo tile (row stride HS_X, plane stride
+  // HS_Y rows), 16-byte units
+  constexpr int UPR = HS_X / 4;
+  constexpr int NU = HP_Z * 15 * UPR;
+  for (int u = tid; u < NU; u += 128) {
+    const int row = u / UPR, xu = u - row * UPR;
+    const int z = row / 15, y = row - z * 15;
+    reinterpret_cast<float4*>(out + (z * HS_Y + y) * HS_X)[xu] = reinterpret_cast<const float4*>(tile + row * HS_X)[xu];
+  }
 }
 
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
@@ -2256,11 +2312,8 @@ __global__ void k_merge_halo(const T* __restrict__ halo, const int* __restrict__
 // dimension (every cell unless a partial last tile sits next to the wrap)
 // issues all its up to 8 P halo loads at once and sums them in the same order
 // as k_merge_halo (z, y, x entries, then slices), so the grid is bitwise the
-// same; 32-bit offsets (the host checks the halo buffer size).  WIN: the
-// parts alternate x windows [0, 16) and [8, 24) of the tile (the matrix-core
-// spreader), zero outside them, so a part is read only where its window
-// covers the cell (the terms skipped are exact zeros).
-template <typename T, int TY, int P, bool WIN = false>
+// same; 32-bit offsets (the host checks the halo buffer size).
+template <typename T, int TY, int P>
 __global__ void __launch_bounds__(256)
     k_merge_halo_p(const T* __restrict__ halo, const int* __restrict__ tabs, SpreadDev<T> a, T* __restrict__ grid) {
   constexpr int HS_Y = HaloShape<TY>::HS_Y;
@@ -2294,8 +2347,7 @@ __global__ void __launch_bounds__(256)
         const int base = (bx[ix] + by[iy] + bz[iz]) * (P * HT) + ux[ix] + uy[iy] + uz[iz];
 #pragma unroll
         for (int sl = 0; sl < P; ++sl) {
-          const bool in = !WIN || ((sl & 1) ? ux[ix] >= 8 : ux[ix] < 16);
-          v[c][sl] = ok && in ? halo[base + sl * HT] : T(0);
+          v[c][sl] = ok ? halo[base + sl * HT] : T(0);
         }
       }
 #pragma unroll
@@ -2678,16 +2730,13 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
     bool done = false;
     if constexpr (sizeof(T) == 4) {
       if (sp.mode == Spreader<T>::HALOM && t8) {
-        if (!sp.halo_buf_clear) {
-          fill_zero(sp.halo_buf.get(), sp.halo_buf.size(), s);
-          sp.halo_buf_clear = true;
-        }
-        hipLaunchKernelGGL(k_spread3_halo_mfma, grd, dim3(64), 0, s, p.scell.get(), p.rec.get(), p.bin_start.get(),
-                           a, sp.halo_buf.get());
+        // one tile per (bin, slice): halo_parts counts 2 windows per slice
+        const dim3 grd2(g.nbins, sp.halo_parts / 2);
+        hipLaunchKernelGGL(k_spread3_halo_mfma2<1024>, grd2, dim3(128), 0, s, p.scell.get(), p.rec.get(),
+                           p.bin_start.get(), a, sp.halo_buf.get());
         done = true;
       }
     }
-    if (!done) sp.halo_buf_clear = false;  // whole tiles written below
     if (done) {
     } else if (sp.mode == Spreader<T>::HALO4) {
       if (t8)
@@ -2708,21 +2757,13 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   const size_t n = (size_t)g.ncells();
   const dim3 grd((unsigned)std::min<size_t>(blocks_for(n, 256), 16384));
   const size_t hel = t8 ? HaloShape<8>::ELEMS : HaloShape<16>::ELEMS;
-  if (sp.halo_parts <= 4 && (size_t)g.nbins * sp.halo_parts * hel < (1u << 31)) {
+  // halo tiles per bin (the matrix-core spreader counts 2 windows per tile)
+  const int tiles = sp.mode == Spreader<T>::HALOM && t8 ? sp.halo_parts / 2 : sp.halo_parts;
+  if (tiles <= 4 && (size_t)g.nbins * tiles * hel < (1u << 31)) {
 #define FRESCO_MERGE_P(TY_, P_)                                                                              \
   hipLaunchKernelGGL((k_merge_halo_p<T, TY_, P_>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, \
                      grid)
-    if (sp.mode == Spreader<T>::HALOM && t8 && (sp.halo_parts == 2 || sp.halo_parts == 4)) {
-      if (sp.halo_parts == 2)
-        hipLaunchKernelGGL((k_merge_halo_p<T, 8, 2, true>), grd, dim3(256), 0, s, sp.halo_buf.get(),
-                           sp.halo_tab.get(), a, grid);
-      else
-        hipLaunchKernelGGL((k_merge_halo_p<T, 8, 4, true>), grd, dim3(256), 0, s, sp.halo_buf.get(),
-                           sp.halo_tab.get(), a, grid);
-      FRESCO_HIP_CHECK(hipGetLastError());
-      return true;
-    }
-    switch (sp.halo_parts * (t8 ? 1 : -1)) {
+    switch (tiles * (t8 ? 1 : -1)) {
       case 1: FRESCO_MERGE_P(8, 1); break;
       case 2: FRESCO_MERGE_P(8, 2); break;
       case 3: FRESCO_MERGE_P(8, 3); break;
@@ -2737,11 +2778,11 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
     return true;
   }
   if (t8)
-    hipLaunchKernelGGL((k_merge_halo<T, 8>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(),
-                       sp.halo_parts, a, grid);
+    hipLaunchKernelGGL((k_merge_halo<T, 8>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), tiles, a,
+                       grid);
   else
-    hipLaunchKernelGGL((k_merge_halo<T, 16>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(),
-                       sp.halo_parts, a, grid);
+    hipLaunchKernelGGL((k_merge_halo<T, 16>), grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), tiles, a,
+                       grid);
   FRESCO_HIP_CHECK(hipGetLastError());
   return true;
 }
@@ -2884,7 +2925,40 @@ __global__ void __launch_bounds__(G8_THREADS)
       if (gz >= a.nf[2]) gz -= a.nf[2];
       return ((int64_t)gz * a.nf[1] + gy) * a.nf[0] + gx;
     };
-    if constexpr (RY != 0) {
+    if constexpr (SRX == 24) {
+      // rows of 24 cells (23 + one that lands in the row's padding) moved as
+      // 16-byte units: the grid row starts at a multiple of the unit (the
+      // launcher checks nf[0] and the tile width), so a unit never straddles
+      // the periodic wrap in x; 32-bit offsets (the launcher checks the grid
+      // size); all of a thread's loads are issued before its LDS stores
+      using V = typename Vec4<float>::type;
+      constexpr int EPU = 16 / sizeof(T);  // cells per unit
+      constexpr int UPR = 24 / EPU;        // units per row
+      constexpr int NU = RY * 15 * UPR;
+      constexpr int PER = (NU + G8_THREADS - 1) / G8_THREADS;
+      const int nu0 = a.nf[0] / EPU, tu0 = t0[0] / EPU;
+      const int plane = a.nf[0] * a.nf[1];
+      V v[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int u = tid + q * G8_THREADS;
+        const int row = u / UPR, xu = u - row * UPR;
+        const int ly = row % RY, lz = row / RY;
+        int gy = t0[1] + ly;
+        if (gy >= a.nf[1]) gy -= a.nf[1];
+        int gz = t0[2] + lz;
+        if (gz >= a.nf[2]) gz -= a.nf[2];
+        int gu = tu0 + xu;
+        if (gu >= nu0) gu -= nu0;
+        const int off = gz * plane + gy * a.nf[0] + gu * EPU;
+        v[q] = (NU % G8_THREADS == 0 || u < NU) ? *reinterpret_cast<const V*>(grid + off) : V{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int u = tid + q * G8_THREADS;
+        if (NU % G8_THREADS == 0 || u < NU) *reinterpret_cast<V*>(reg + u * EPU) = v[q];
+      }
+    } else if constexpr (RY != 0) {
       // fixed region (RY x 15 rows, the launcher checks rz): every load of
       // a thread is issued before its LDS stores (one memory latency per
       // workgroup instead of one per row)
@@ -3079,7 +3153,26 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   // 0.112 / 0.112 ms for 1024 / 2048 / 3072 / 4096)
   const int chunks = std::max(1, std::min(8, (4096 + g.nbins - 1) / g.nbins));
   const dim3 grd(g.nbins, chunks);
-  if (a.srx == 23 && a.ry == 15 && a.rx == 23 && a.rz == 15)
+  // 16-byte row units: whole units per grid row, the tile's first cell on a
+  // unit, one wrap per dimension at most, 32-bit offsets, an aligned grid
+  static const bool units_on = [] {
+    const char* e = std::getenv("FRESCO_INTERP_UNITS");
+    return !e || std::atoi(e) != 0;
+  }();
+  const int epu = 16 / (int)sizeof(T);
+  const bool units = units_on && a.rx == 23 && a.rz == 15 && (a.ry == 15 || a.ry == 23) && g.nf[0] % epu == 0 &&
+                     g.tile[0] % epu == 0 && g.nf[0] >= 24 && g.nf[1] >= a.ry && g.nf[2] >= 15 &&
+                     g.ncells() < (1ll << 31) && ((uintptr_t)grid % 16) == 0;
+  if (units) {
+    lds = sizeof(T) * (size_t)24 * a.ry * a.rz;
+    a.srx = 24;
+    if (a.ry == 15)
+      hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 24, 15>), grd, dim3(G8_THREADS), lds, s, p.scell.get(),
+                         p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+    else
+      hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 24, 23>), grd, dim3(G8_THREADS), lds, s, p.scell.get(),
+                         p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+  } else if (a.srx == 23 && a.ry == 15 && a.rx == 23 && a.rz == 15)
     hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 23, 15>), grd, dim3(G8_THREADS), lds, s, p.scell.get(), p.rec.get(),
                        p.idx.get(), p.bin_start.get(), grid, a, out);
   else if (a.srx == 23 && a.ry == 23 && a.rx == 23 && a.rz == 15)
