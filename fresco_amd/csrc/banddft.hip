@@ -9,6 +9,7 @@
 #include "banddft.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -491,10 +492,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// W[k][j] = wt(j') exp(sign 2 pi i k' j' / n) with k' = k + koff, j' = j + joff
+// W[k][j] = wt(j') exp(sign 2 pi i k' j' / n) with k' = k + koff, j' = j + joff,
+// stored at [k * stride + j] of a zero-padded [prow][stride] array (prow >= K,
+// stride >= J; stride = J, prow = K: the plain [K][J] matrix)
 template <typename T>
-void twiddles(DeviceBuffer<Cplx<T>>& buf, int K, int koff, int J, int joff, int n, int sign, bool weight_kx) {
-  std::vector<Cplx<T>> h((size_t)K * J);
+void twiddles(DeviceBuffer<Cplx<T>>& buf, int K, int koff, int J, int joff, int n, int sign, bool weight_kx,
+              int prow = 0, int stride = 0) {
+  if (prow == 0) prow = K;
+  if (stride == 0) stride = J;
+  std::vector<Cplx<T>> h((size_t)prow * stride, Cplx<T>{T(0), T(0)});
   for (int k = 0; k < K; ++k)
     for (int j = 0; j < J; ++j) {
       const long long kk = k + koff, jj = j + joff;
@@ -502,10 +508,423 @@ void twiddles(DeviceBuffer<Cplx<T>>& buf, int K, int koff, int J, int joff, int 
       if (r < 0) r += n;
       const double ang = sign * 2.0 * M_PI * (double)r / (double)n;
       const double wt = weight_kx ? (jj == 0 ? 1.0 : 2.0) : 1.0;
-      h[(size_t)k * J + j] = Cplx<T>{(T)(wt * std::cos(ang)), (T)(wt * std::sin(ang))};
+      h[(size_t)k * stride + j] = Cplx<T>{(T)(wt * std::cos(ang)), (T)(wt * std::sin(ang))};
     }
   buf.allocate(h.size());
   buf.upload(h.data(), h.size());
+}
+
+
+
+// ---------------------------------------------------------------------------
+// Fused FP32 form (three launches per energy + gradient evaluation, the loss
+// inside the middle one):
+//   k_band_plane_fwd  per z plane: U1[y][kx] = sum_x g[z][y][x] Wx[kx][x],
+//                     then U2[z][ky][kx] = sum_y Wy[ky][y] U1[y][kx]
+//   k_band_zloss      per tile of 16 band columns (ky, kx): S[kz][col] =
+//                     sum_z Wz[kz][z] U2[z][col], the loss of the pairs whose
+//                     representative lies in the tile and their pullback
+//                     values X (in LDS), then Y1[z][col] = sum_kz Wzi[z][kz]
+//                     X[kz][col]; the kx = 0 columns ky and -ky share a tile,
+//                     so every X a tile needs is written in it
+//   k_band_plane_inv  per z plane: Y2[y][kx] = sum_ky Wyi[y][ky] Y1[z][ky][kx],
+//                     then grid[z][y][x] = Re sum_kx Wxi[x][kx] Y2[y][kx]
+// Every operand a workgroup reads more than once is staged in LDS with all
+// of its loads issued together (one memory latency per stage); products are
+// v_mfma_f32_16x16x4f32 tiles as in k_band_mfma (a complex product is four
+// real ones, two for a real input or output).
+// ---------------------------------------------------------------------------
+
+constexpr int BP_T = 512;   // threads of the plane kernels (8 wavefronts)
+constexpr int BP_YG = 4;    // y tiles per inverse plane workgroup
+constexpr int BZ_T = 512;   // threads of the z-pass + loss kernel (8 wavefronts)
+constexpr int BZ_COLS = 16;  // band columns per z-pass workgroup
+constexpr int BZ_KROWS = 80;  // kz rows of the z-pass workgroup's S / X tiles (band <= 32)
+
+// LDS row lengths, in complex entries, for the two read patterns of a
+// 16 x 16 x 4 product (each half wavefront reads 256 bytes):
+//   B operand, lane (n, q) reads [4 s + q][n]: rows 16 mod 32 entries apart
+__host__ __device__ constexpr int bd_bstride(int cols) {
+  return ((cols + 15) / 16 * 16) % 32 == 0 ? (cols + 15) / 16 * 16 + 16 : (cols + 15) / 16 * 16;
+}
+//   A operand, lane (m, q) reads [m][4 s + q]: rows 2 mod 32 entries apart
+__host__ __device__ constexpr int bd_astride(int cols) { return (cols + 31) / 32 * 32 + 2; }
+
+__device__ __forceinline__ void cmfma(const Cplx<float>& a, const Cplx<float>& b, mf_f4& cr, mf_f4& ci) {
+  cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.re, cr, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.im, -b.im, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.im, ci, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.im, b.re, ci, 0, 0, 0);
+}
+
+// Row lengths of the operand arrays (complex entries unless noted); every
+// contracted index is zero-padded to a multiple of 16 (4 steps of 4), so the
+// product loops read 4 steps' operands at once without bounds checks.
+struct BandShape {
+  int nx, ny, nz, nb, nk;
+  int nyR, nxR, nzR, nbR, nkR;  // rounded up to 16
+  int PX;                       // plane row (floats)
+  int SWX;                      // forward plane: Wx [nbR][SWX] (a workgroup stages one 16-row tile)
+  int SWZI;                     // z pass: Wzi [nzR][SWZI]
+  int SYT, SY2, SWYI, SWXI;     // inverse plane: Y1 [nkR][SYT], Y2 [nyR][SY2], Wyi [nyR][SWYI], Wxi [nxR][SWXI]
+  __host__ __device__ BandShape(int nx_, int ny_, int nz_, int nb_, int nk_)
+      : nx(nx_), ny(ny_), nz(nz_), nb(nb_), nk(nk_) {
+    nyR = (ny + 15) / 16 * 16;
+    nxR = (nx + 15) / 16 * 16;
+    nzR = (nz + 15) / 16 * 16;
+    nbR = (nb + 15) / 16 * 16;
+    nkR = (nk + 15) / 16 * 16;
+    PX = mf_jp(nx);
+    SWX = bd_astride(nx);
+    SWZI = bd_astride(nk);
+    SYT = bd_bstride(nb);
+    SY2 = bd_astride(nb);
+    SWYI = bd_astride(nk);
+    SWXI = bd_astride(nb);
+  }
+  __host__ __device__ size_t fwd_lds() const { return (size_t)nyR * PX * 4 + (size_t)nyR * 16 * 8 + (size_t)16 * SWX * 8; }
+  __host__ __device__ size_t inv_lds() const {
+    return (size_t)nkR * SYT * 8 + (size_t)BP_YG * 16 * (SY2 + SWYI) * 8 + (size_t)nxR * SWXI * 8;
+  }
+};
+
+// n 16-byte units, batches of NT x U (each batch's loads issued together)
+template <int NT, int U>
+__device__ __forceinline__ void copy_units(void* dst_, const void* __restrict__ src_, int n) {
+  float4* dst = static_cast<float4*>(dst_);
+  const float4* src = static_cast<const float4*>(src_);
+  for (int b0 = 0; b0 < n; b0 += NT * U) {
+    // clamped, unconditional loads keep the batch in registers
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[min(b0 + u * NT + (int)threadIdx.x, n - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = b0 + u * NT + (int)threadIdx.x;
+      if (e < n) dst[e] = v[u];
+    }
+  }
+}
+
+// grid [nz][ny][nx] (real), Wx padded [nbR][SWX], Wy [nk][ny] -> U2 [nz][nk][nb]
+__global__ void __launch_bounds__(BP_T)
+    k_band_plane_fwd(const float* __restrict__ grid, BandShape sh, const Cplx<float>* __restrict__ Wxp,
+                     const Cplx<float>* __restrict__ Wy, Cplx<float>* __restrict__ U2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  // workgroup (z, nt): plane z, kx tile nt (the x pass of one kx tile
+  // feeds the y pass of the same tile only)
+  const int z = blockIdx.x, nt = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  const int nx = sh.nx, ny = sh.ny, nb = sh.nb, nk = sh.nk;
+  const int nyT = sh.nyR / 16, nkT = sh.nkR / 16;
+  const int PX = sh.PX, SB = 16, SW = sh.SWX;
+  float* plane = reinterpret_cast<float*>(smem);                                 // [nyR][PX]
+  Cplx<float>* u1 = reinterpret_cast<Cplx<float>*>(smem + (size_t)sh.nyR * PX * 4);  // [nyR][16]
+  Cplx<float>* wx = u1 + (size_t)sh.nyR * SB;                                      // [16][SW]: tile nt
+  {
+    // the plane: one wavefront per row, lanes over columns (zero padding),
+    // every load issued before the stores; the padded twiddles as units
+    constexpr int MAXR = 128 / (BP_T / 64), NCH = 3;
+    const float* gp = grid + (int64_t)z * ny * nx;
+    float v[MAXR][NCH];
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int r = wv + i * (BP_T / 64), cc = lane + 64 * c;
+        v[i][c] = (r < ny && cc < nx) ? gp[r * nx + cc] : 0.f;
+      }
+    copy_units<BP_T, 8>(wx, Wxp + (size_t)nt * 16 * SW, 16 * SW / 2);
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int r = wv + i * (BP_T / 64), cc = lane + 64 * c;
+        if (r < sh.nyR && cc < PX) plane[r * PX + cc] = v[i][c];
+      }
+  }
+  __syncthreads();
+  // x: U1[y][kx], A = plane rows (y), B[x][kx] = Wx[kx][x]; 4 steps a round
+  const int KX4 = (nx + 15) / 16;
+  for (int mt = wv; mt < nyT; mt += BP_T / 64) {
+    const float* arow = plane + (mt * 16 + m) * PX + q;
+    const Cplx<float>* brow = wx + m * SW + q;
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < KX4; ++s4) {
+      float a[4];
+      Cplx<float> b[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = arow[16 * s4 + 4 * j];
+        b[j] = brow[16 * s4 + 4 * j];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j].re, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j].im, ci, 0, 0, 0);
+      }
+    }
+    // D[y][kx]: register r of lane (kx = m, q) is row 4 q + r
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u1[(mt * 16 + 4 * q + r) * SB + m] = Cplx<float>{cr[r], ci[r]};
+  }
+  __syncthreads();
+  // y: U2[ky][kx], A[ky][y] = Wy (all of a tile's steps loaded first), B = U1
+  const int KY = (ny + 3) / 4;
+  for (int mt = wv; mt < nkT; mt += BP_T / 64) {
+    const int ky = mt * 16 + m;
+    Cplx<float> aw[FZ_MAXS];
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int y = 4 * st + q;
+      aw[st] = (st < KY && ky < nk && y < ny) ? Wy[(int64_t)ky * ny + y] : Cplx<float>{0.f, 0.f};
+    }
+    const Cplx<float>* bcol = u1 + q * SB + m;
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s4 = 0; s4 < FZ_MAXS / 4; ++s4) {
+      if (4 * s4 < KY) {
+        Cplx<float> b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = bcol[(16 * s4 + 4 * j) * SB];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cmfma(aw[4 * s4 + j], b[j], cr, ci);
+      }
+    }
+    const int kx = nt * 16 + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kyo = mt * 16 + 4 * q + r;
+      if (kyo < nk && kx < nb) U2[((int64_t)z * nk + kyo) * nb + kx] = Cplx<float>{cr[r], ci[r]};
+    }
+  }
+}
+
+// U2 [nz][nk][nb] -> S (compact band spectrum), the tile's pair energies and,
+// MODE 1, Y1 [nz][nk][nb]; tcol [tiles][16]: column ky * nb + kx, or -1;
+// Wzip: Wzi padded [nzR][SWZI]
+template <int MODE>
+__global__ void __launch_bounds__(BZ_T)
+    k_band_zloss(const Cplx<float>* __restrict__ U2, BandShape sh, const int* __restrict__ tcol,
+                 const Cplx<float>* __restrict__ Wz, const Cplx<float>* __restrict__ Wzip, PairDev pd,
+                 BandLossDev bl, double norm, double gscale, Cplx<float>* __restrict__ S,
+                 Cplx<float>* __restrict__ Y1, double* __restrict__ partial, unsigned* __restrict__ ticket,
+                 double* __restrict__ e_out, double* __restrict__ e_host) {
+  __shared__ __align__(16) Cplx<float> ut[128][BZ_COLS];          // U2 columns [z][slot]
+  __shared__ __align__(16) Cplx<float> sx[2][BZ_KROWS][BZ_COLS];  // S and X [kz][slot]
+  extern __shared__ __align__(16) unsigned char smem[];           // Wzi [nzR][SWZI] (MODE 1)
+  __shared__ double red[BZ_T];
+  __shared__ int cols[BZ_COLS];
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  const int nz = sh.nz, nk = sh.nk;
+  const int64_t plane = (int64_t)nk * sh.nb;
+  const int SW = sh.SWZI;
+  Cplx<float>* wzi = reinterpret_cast<Cplx<float>*>(smem);
+  // the tables of each thread's first pair and the forward z pass's first
+  // twiddle tile are loaded before anything waits
+  const int p0 = bl.pstart[tile], p1 = bl.pstart[tile + 1];
+  const int KZ = (nz + 3) / 4, nkT = sh.nkR / 16;
+  PairVals pv0{};
+  int lc0 = 0, mr0 = -1;
+  if (p0 + tid < p1) {
+    pv0 = pair_vals(pd, p0 + tid);
+    lc0 = bl.loc[p0 + tid];
+    mr0 = bl.mir[p0 + tid];
+  }
+  Cplx<float> aw[FZ_MAXS];
+  auto load_wz = [&](int mt) {
+    const int kz = mt * 16 + m;
+#pragma unroll
+    for (int st = 0; st < FZ_MAXS; ++st) {
+      const int zz = 4 * st + q;
+      aw[st] = (st < KZ && kz < nk && zz < nz) ? Wz[(int64_t)kz * nz + zz] : Cplx<float>{0.f, 0.f};
+    }
+  };
+  load_wz(wv);
+  {
+    constexpr int PU = 128 * BZ_COLS / BZ_T;
+    const int* tc = tcol + tile * BZ_COLS;
+    Cplx<float> v[PU];
+#pragma unroll
+    for (int u = 0; u < PU; ++u) {
+      const int e = tid + u * BZ_T;
+      const int zz = e / BZ_COLS, col = tc[e % BZ_COLS];
+      v[u] = (zz < nz && col >= 0) ? U2[zz * plane + col] : Cplx<float>{0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < PU; ++u) {
+      const int e = tid + u * BZ_T;
+      ut[e / BZ_COLS][e % BZ_COLS] = v[u];
+    }
+    for (int e = tid; e < 2 * BZ_KROWS * BZ_COLS; e += BZ_T) (&sx[0][0][0])[e] = Cplx<float>{0.f, 0.f};
+    if (MODE > 0) copy_units<BZ_T, 8>(wzi, Wzip, sh.nzR * SW / 2);
+    if (tid < BZ_COLS) cols[tid] = tc[tid];
+  }
+  __syncthreads();
+  // forward z: S[kz][slot], A[kz][z] = Wz (a tile's steps loaded first), B = ut
+  for (int mt = wv; mt < nkT; mt += BZ_T / 64) {
+    if (mt != wv) load_wz(mt);
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s4 = 0; s4 < FZ_MAXS / 4; ++s4) {
+      if (4 * s4 < KZ) {
+        Cplx<float> b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = ut[16 * s4 + 4 * j + q][m];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cmfma(aw[4 * s4 + j], b[j], cr, ci);
+      }
+    }
+    const int col = cols[m];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kzo = mt * 16 + 4 * q + r;
+      const Cplx<float> sv{cr[r], ci[r]};
+      sx[0][kzo][m] = sv;
+      if (kzo < nk && col >= 0) S[kzo * plane + col] = sv;
+    }
+  }
+  __syncthreads();
+  // the loss of the tile's pairs, each thread's in index order
+  double e = 0.0;
+  for (int i = p0 + tid; i < p1; i += BZ_T) {
+    const bool first = i == p0 + tid;
+    const PairVals pv = first ? pv0 : pair_vals(pd, i);
+    const int lc = first ? lc0 : bl.loc[i];
+    double xr, xi;
+    e += pair_loss(pv, sx[0][lc >> 4][lc & 15], norm, gscale, MODE > 0, xr, xi);
+    // k = 0 moves no point: it stays off the pullback (as in k_nuwu_loss)
+    if (MODE > 0 && !pv.self) {
+      sx[1][lc >> 4][lc & 15] = Cplx<float>{(float)xr, (float)xi};
+      const int mr = first ? mr0 : bl.mir[i];
+      if (mr >= 0) sx[1][mr >> 4][mr & 15] = Cplx<float>{(float)xr, (float)(-xi)};
+    }
+  }
+  red[tid] = e;
+  __syncthreads();
+  for (int hh = BZ_T / 2; hh >= 1; hh >>= 1) {
+    if (tid < hh) red[tid] += red[tid + hh];
+    __syncthreads();
+  }
+  if (tid == 0) partial[tile] = red[0];
+  if (MODE > 0) {
+    // inverse z: Y1[z][slot], A[z][kz] = Wzi (LDS), B = X; 4 steps a round
+    const int KK4 = (nk + 15) / 16, nzT = sh.nzR / 16;
+    for (int mt = wv; mt < nzT; mt += BZ_T / 64) {
+      const Cplx<float>* arow = wzi + (mt * 16 + m) * SW + q;
+      mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+      for (int s4 = 0; s4 < KK4; ++s4) {
+        Cplx<float> a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          a[j] = arow[16 * s4 + 4 * j];
+          b[j] = sx[1][16 * s4 + 4 * j + q][m];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cmfma(a[j], b[j], cr, ci);
+      }
+      const int col = cols[m];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int zo = mt * 16 + 4 * q + r;
+        if (zo < nz && col >= 0) Y1[zo * plane + col] = Cplx<float>{cr[r], ci[r]};
+      }
+    }
+  }
+  // the last workgroup sums the tiles' energies in their fixed order
+  if (last_block_arrived(ticket, gridDim.x)) block_sum_partials<BZ_T>(partial, gridDim.x, e_out, e_host);
+}
+
+// Y1 [nz][nk][nb], Wyi padded [nyR][SWYI], Wxi padded [nxR][SWXI] (kx > 0
+// weighted twice) -> grid
+__global__ void __launch_bounds__(BP_T)
+    k_band_plane_inv(const Cplx<float>* __restrict__ Y1, BandShape sh, const Cplx<float>* __restrict__ Wyip,
+                     const Cplx<float>* __restrict__ Wxip, float* __restrict__ grid) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  // workgroup (z, g): plane z, y tiles BP_YG g .. BP_YG g + BP_YG - 1 (both
+  // passes keep y as the row index, so the groups are independent)
+  const int z = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  const int nx = sh.nx, ny = sh.ny, nb = sh.nb, nk = sh.nk;
+  const int yt0 = BP_YG * blockIdx.y;
+  const int nyT = min(BP_YG, sh.nyR / 16 - yt0), nbT = sh.nbR / 16, nxT = sh.nxR / 16;
+  const int SB = sh.SYT, SA = sh.SY2, SWY = sh.SWYI, SWX = sh.SWXI;
+  Cplx<float>* yt = reinterpret_cast<Cplx<float>*>(smem);  // [nkR][SB]: Y1[z]
+  Cplx<float>* y2 = yt + (size_t)sh.nkR * SB;              // [BP_YG * 16][SA]: the group's rows
+  Cplx<float>* wyi = y2 + (size_t)BP_YG * 16 * SA;         // [BP_YG * 16][SWY]
+  Cplx<float>* wxi = wyi + (size_t)BP_YG * 16 * SWY;       // [nxR][SWX]
+  {
+    // Y1[z]: one wavefront per row, lanes over columns (zero padding)
+    constexpr int MAXR = 80 / (BP_T / 64);
+    const Cplx<float>* yp = Y1 + (int64_t)z * nk * nb;
+    Cplx<float> v[MAXR];
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+      const int r = wv + i * (BP_T / 64);
+      v[i] = (r < nk && lane < nb) ? yp[r * nb + lane] : Cplx<float>{0.f, 0.f};
+    }
+    copy_units<BP_T, 8>(wyi, Wyip + (size_t)yt0 * 16 * SWY, nyT * 16 * SWY / 2);
+    copy_units<BP_T, 8>(wxi, Wxip, sh.nxR * SWX / 2);
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+      const int r = wv + i * (BP_T / 64);
+      if (r < sh.nkR && lane < SB) yt[r * SB + lane] = v[i];
+    }
+  }
+  __syncthreads();
+  // y: Y2[y][kx], A[y][ky] = Wyi, B[ky][kx] = Y1[z]; 4 steps a round
+  const int KK4 = (nk + 15) / 16;
+  for (int t = wv; t < nyT * nbT; t += BP_T / 64) {
+    const int mt = t / nbT, nt = t - mt * nbT;
+    const Cplx<float>* arow = wyi + (mt * 16 + m) * SWY + q;
+    const Cplx<float>* bcol = yt + q * SB + nt * 16 + m;
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < KK4; ++s4) {
+      Cplx<float> a[4], b[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = arow[16 * s4 + 4 * j];
+        b[j] = bcol[(16 * s4 + 4 * j) * SB];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cmfma(a[j], b[j], cr, ci);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y2[(mt * 16 + 4 * q + r) * SA + nt * 16 + m] = Cplx<float>{cr[r], ci[r]};
+  }
+  __syncthreads();
+  // x: grid[y][x] = Re sum_kx Y2[y][kx] Wxi[x][kx], A = Y2, B[kx][x] = Wxi[x][kx]
+  float* gp = grid + ((int64_t)z * ny + yt0 * 16) * nx;
+  const int KB4 = (nb + 15) / 16;
+  for (int t = wv; t < nyT * nxT; t += BP_T / 64) {
+    const int mt = t / nxT, nt = t - mt * nxT;
+    const Cplx<float>* arow = y2 + (mt * 16 + m) * SA + q;
+    const Cplx<float>* brow = wxi + (nt * 16 + m) * SWX + q;
+    mf_f4 cr = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < KB4; ++s4) {
+      Cplx<float> a[4], b[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = arow[16 * s4 + 4 * j];
+        b[j] = brow[16 * s4 + 4 * j];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].re, b[j].re, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].im, -b[j].im, cr, 0, 0, 0);
+      }
+    }
+    const int x = nt * 16 + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int yo = mt * 16 + 4 * q + r;
+      if (yt0 * 16 + yo < ny && x < nx) gp[yo * nx + x] = cr[r];
+    }
+  }
 }
 
 }  // namespace
@@ -528,6 +947,80 @@ void BandDft<T>::init(const int* nf, int band) {
   twiddles(wai_, nf_[0], 0, nb_, 0, nf_[0], 1, true);      // x x kx, kx > 0 twice
   u1_.allocate((size_t)nf_[2] * nf_[1] * nb_);
   u2_.allocate((size_t)nf_[2] * nk * nb_);
+  const char* env = std::getenv("FRESCO_BANDDFT_FUSED");
+  const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, nk);
+  fused_ = std::is_same<T, float>::value && !(env && env[0] == '0') && sh.nkR <= BZ_KROWS && sh.nzR <= 128 &&
+           sh.fwd_lds() <= 160 * 1024 && sh.inv_lds() <= 160 * 1024;
+  if (fused_) {
+    // the twiddles the fused kernels stage, in their zero-padded LDS layouts
+    twiddles(wxp_, nb_, 0, nf_[0], 0, nf_[0], -1, false, sh.nbR, sh.SWX);
+    twiddles(wzip_, nf_[2], 0, nk, -band, nf_[2], 1, false, sh.nzR, sh.SWZI);
+    twiddles(wyip_, nf_[1], 0, nk, -band, nf_[1], 1, false, sh.nyR, sh.SWYI);
+    twiddles(wxip_, nf_[0], 0, nb_, 0, nf_[0], 1, true, sh.nxR, sh.SWXI);
+    // column list: kx = 0 first, ky = 0 alone and then ky, -ky side by side
+    // (never split over two tiles), then kx > 0 row by row; tiles of 16
+    std::vector<int> list;
+    auto push = [&](int c0, int c1) {
+      if (c1 >= 0 && list.size() % 16 == 15) list.push_back(-1);
+      list.push_back(c0);
+      if (c1 >= 0) list.push_back(c1);
+    };
+    auto col = [&](int ky, int kx) { return (ky + band) * nb_ + kx; };
+    push(col(0, 0), -1);
+    for (int j = 1; j <= band; ++j) push(col(j, 0), col(-j, 0));
+    for (int ky = -band; ky <= band; ++ky)
+      for (int kx = 1; kx < nb_; ++kx) push(col(ky, kx), -1);
+    while (list.size() % 16) list.push_back(-1);
+    ntiles_ = (int)(list.size() / 16);
+    col_pos_.assign((size_t)nk * nb_, -1);
+    for (size_t i = 0; i < list.size(); ++i)
+      if (list[i] >= 0) col_pos_[list[i]] = (int)i;
+    tcol_.allocate(list.size());
+    tcol_.upload(list.data(), list.size());
+  }
+}
+
+template <typename T>
+void BandDft<T>::forward_planes(const T* grid, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, 2 * B_ + 1);
+    hipLaunchKernelGGL(k_band_plane_fwd, dim3(nf_[2], sh.nbR / 16), dim3(BP_T), sh.fwd_lds(), s, grid, sh,
+                       wxp_.get(), wb_.get(), u2_.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+  } else {
+    throw RuntimeFailure("band transform: the fused form is FP32 only");
+  }
+}
+
+template <typename T>
+void BandDft<T>::zloss(int mode, const PairDev& pd, const BandLossDev& bl, double norm, double gscale, Cplx<T>* S,
+                       double* partial, unsigned* ticket, double* e_out, double* e_host, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, 2 * B_ + 1);
+    if (mode == 0) {
+      hipLaunchKernelGGL(k_band_zloss<0>, dim3(ntiles_), dim3(BZ_T), 0, s, u2_.get(), sh, tcol_.get(), wc_.get(),
+                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial, ticket, e_out, e_host);
+    } else {
+      const size_t lds = sizeof(Cplx<float>) * (size_t)sh.nzR * sh.SWZI;
+      hipLaunchKernelGGL(k_band_zloss<1>, dim3(ntiles_), dim3(BZ_T), lds, s, u2_.get(), sh, tcol_.get(), wc_.get(),
+                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial, ticket, e_out, e_host);
+    }
+    FRESCO_HIP_CHECK(hipGetLastError());
+  } else {
+    throw RuntimeFailure("band transform: the fused form is FP32 only");
+  }
+}
+
+template <typename T>
+void BandDft<T>::inverse_planes(T* grid, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, 2 * B_ + 1);
+    hipLaunchKernelGGL(k_band_plane_inv, dim3(nf_[2], (sh.nyR / 16 + BP_YG - 1) / BP_YG), dim3(BP_T), sh.inv_lds(), s,
+                       u1_.get(), sh, wyip_.get(), wxip_.get(), grid);
+    FRESCO_HIP_CHECK(hipGetLastError());
+  } else {
+    throw RuntimeFailure("band transform: the fused form is FP32 only");
+  }
 }
 
 template <typename T>

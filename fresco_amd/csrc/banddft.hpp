@@ -21,10 +21,21 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "nufft.hpp"
+#include "nuwu_loss.hpp"
 
 namespace fresco_hip {
+
+// The pairs of the loss grouped by the z-pass tile of their representative
+// (fused FP32 form): pairs pstart[t] .. pstart[t + 1] - 1 belong to tile t;
+// loc / mir: kz index * 16 + slot of k and of -k (kx = 0, k != 0; else -1)
+struct BandLossDev {
+  const int* pstart;
+  const int* loc;
+  const int* mir;
+};
 
 template <typename T>
 class BandDft {
@@ -49,12 +60,38 @@ class BandDft {
   }
   int band() const { return B_; }
 
+  // Fused FP32 form, three launches per evaluation: forward_planes (x and y
+  // passes of each z plane), zloss (z pass of each tile of 16 band columns
+  // (ky, kx), the loss of the pairs whose representative lies in the tile,
+  // mode 1: the pullback values and the inverse z pass) and inverse_planes
+  // (y and x passes of each z plane).  The spectrum S is written in the
+  // compact layout; the pullback values exist only inside zloss.
+  // (FRESCO_BANDDFT_FUSED=0: the four-launch form.)
+  bool fused() const { return fused_; }
+  int ntiles() const { return ntiles_; }
+  // z-pass tile and slot of band column (ky, kx); the kx = 0 columns ky and
+  // -ky share a tile
+  void column_slot(int ky, int kx, int* tile, int* slot) const {
+    const int c = (ky + B_) * nb_ + kx;
+    *tile = col_pos_[c] / 16;
+    *slot = col_pos_[c] % 16;
+  }
+  void forward_planes(const T* grid, hipStream_t s);
+  void zloss(int mode, const PairDev& pd, const BandLossDev& bl, double norm, double gscale, Cplx<T>* S,
+             double* partial, unsigned* ticket, double* e_out, double* e_host, hipStream_t s);
+  void inverse_planes(T* grid, hipStream_t s);
+
  private:
   int nf_[3] = {0, 0, 0};
   int B_ = -1, nb_ = 0;
   // twiddles of the six passes, [K][J] with J the contracted length
   DeviceBuffer<Cplx<T>> wa_, wb_, wc_, wci_, wbi_, wai_;
   DeviceBuffer<Cplx<T>> u1_, u2_;  // intermediates (the inverse reuses them)
+  bool fused_ = false;
+  DeviceBuffer<Cplx<T>> wxp_, wzip_, wyip_, wxip_;  // fused form: twiddles in padded LDS layouts
+  int ntiles_ = 0;
+  std::vector<int> col_pos_;  // band column (ky + B) nb + kx -> tile * 16 + slot
+  DeviceBuffer<int> tcol_;    // [tiles][16]: column, or -1
 };
 
 }  // namespace fresco_hip

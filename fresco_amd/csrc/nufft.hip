@@ -3925,12 +3925,13 @@ void fill_zero(T* p, size_t n, hipStream_t s) {
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
-  block_sum_partials(part, n, out, nullptr);
+__global__ void k_reduce_partials(const double* __restrict__ part, int n, double* __restrict__ out,
+                                  double* __restrict__ out_host) {
+  block_sum_partials(part, n, out, out_host);
 }
 
-void reduce_partials(const double* partials, int n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, partials, n, out);
+void reduce_partials(const double* partials, int n, double* out, hipStream_t s, double* out_host) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, partials, n, out, out_host);
   FRESCO_HIP_CHECK(hipGetLastError());
 }
 

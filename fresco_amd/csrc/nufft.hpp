@@ -193,17 +193,19 @@ class FftPlan {
 template <typename T>
 void fill_zero(T* p, size_t n, hipStream_t s);
 
-// Deterministic FP64 sum of partials[n] -> *out (device), Kahan-compensated.
-void reduce_partials(const double* partials, int n, double* out, hipStream_t s);
+// Deterministic FP64 sum of partials[n] -> *out (device), Kahan-compensated,
+// and -> *out_host (device-visible pinned host memory) when not null.
+void reduce_partials(const double* partials, int n, double* out, hipStream_t s, double* out_host = nullptr);
 
-// The same compensated, fixed-order sum by one 256-thread workgroup (every
-// thread calls it): part[n] -> *out, and *out_host (device-visible pinned
-// host memory) when not null.
+// The same compensated, fixed-order sum by one workgroup of NT threads
+// (every thread calls it; NT a power of two): part[n] -> *out, and
+// *out_host (device-visible pinned host memory) when not null.
+template <int NT = 256>
 __device__ __forceinline__ void block_sum_partials(const double* part, int n, double* out, double* out_host) {
-  __shared__ double ss[256];
-  __shared__ double cc[256];
+  __shared__ double ss[NT];
+  __shared__ double cc[NT];
   double sum = 0.0, comp = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += NT) {
     double y = part[i] - comp;
     double t = sum + y;
     comp = (t - sum) - y;
@@ -212,7 +214,7 @@ __device__ __forceinline__ void block_sum_partials(const double* part, int n, do
   ss[threadIdx.x] = sum;
   cc[threadIdx.x] = comp;
   __syncthreads();
-  for (int h = 128; h >= 1; h >>= 1) {
+  for (int h = NT / 2; h >= 1; h >>= 1) {
     if ((int)threadIdx.x < h) {
       // two compensated partial sums: add with a TwoSum and keep the error
       double a = ss[threadIdx.x], b = ss[threadIdx.x + h];

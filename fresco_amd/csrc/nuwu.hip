@@ -23,25 +23,13 @@
 #include <string>
 
 #include "banddft.hpp"
+#include "nuwu_loss.hpp"
 #include "nufft.hpp"
 #include "potential.hpp"
 
 namespace fresco_hip {
 
 namespace {
-
-struct PairDev {
-  const int* h;      // half-spectrum index of k (kx >= 0 representative)
-  const int* hneg;   // half-spectrum index of -k when kx == 0 and k != 0, else -1
-  const int* ik;     // flat mode index of k (reference order)
-  const int* imk;    // flat mode index of -k
-  const int* kv;     // k components [npairs*3]
-  const double* Vk;
-  const double* Vm;
-  const double* Sk;
-  const double* Sm;
-  const double* ipsi;  // 1 / Psi(k)
-};
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
@@ -53,41 +41,11 @@ __global__ void __launch_bounds__(256)
   const int i = blockIdx.x * 256 + threadIdx.x;
   double e = 0.0;
   if (i < npairs) {
-    const Cplx<T> sv = S[pd.h[i]];
-    const double ip = pd.ipsi[i];
-    const double re = (double)sv.re * ip;   // rho_k = conj(S_k) / Psi(k)
-    const double im = -(double)sv.im * ip;
-    const double s = (re * re + im * im) / norm;  // nuwu.hpp:262-264
-    const double Sk = pd.Sk[i], Vk = pd.Vk[i];
-    double D = s - Sk;
-    if (Sk != 0) D /= Sk;  // nuwu.hpp:265-268
-    e = Vk * D * D;        // nuwu.hpp:270-271
-    double gk = Vk * D;
-    if (Sk != 0) gk /= Sk;  // nuwu.hpp:272-276
-    const bool self = pd.ik[i] == pd.imk[i];
-    double gm = gk;
-    if (!self) {
-      const double Sm = pd.Sm[i], Vm = pd.Vm[i];
-      double Dm = s - Sm;
-      if (Sm != 0) Dm /= Sm;
-      e += Vm * Dm * Dm;
-      gm = Vm * Dm;
-      if (Sm != 0) gm /= Sm;
-    }
+    double xr, xi;
+    const PairVals pv = pair_vals(pd, i);
+    const bool self = pv.self;
+    e = pair_loss(pv, S[pd.h[i]], norm, gscale, MODE > 0, xr, xi);
     if (MODE > 0) {
-      // H_k = (factor_k + conj(factor_-k)) / 2 with factor = gscale * g * rho
-      double hr, hi;
-      if (self) {
-        hr = gscale * gk * re;
-        hi = 0.0;
-      } else {
-        const double gg = 0.5 * gscale * (gk + gm);
-        hr = gg * re;
-        hi = gg * im;
-      }
-      // C2R input X_k = conj(H_k) / Psi(k): the backward FFT then yields
-      // sum_k H_k / Psi(k) e^{-i k.l h} (type-2 sign -1, nuwu.hpp:51).
-      const double xr = hr * ip, xi = -hi * ip;
       const int h = pd.h[i], hn = pd.hneg[i];
       if (MODE == 1) {
         // k = 0 (the only self-paired mode) moves no point: leaving it out
@@ -262,6 +220,15 @@ class NUwU final : public Potential {
       pts_.set(x, map_e_, c_.get(), s);
     }
     spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
+    if (fz_) {
+      {
+        ProfScope p(ST_FFT_FWD, s);
+        bdft_.forward_planes(grid_.get(), s);
+      }
+      ProfScope p(ST_LOSS, s);
+      launch_zloss(0, s, e_dev_.get(), host_.dev());
+      return host_.wait(s);
+    }
     {
       ProfScope p(ST_FFT_FWD, s);
       forward_spectrum(s);
@@ -407,6 +374,17 @@ class NUwU final : public Potential {
       pts_.set(x, map_g_, c_.get(), s);
     }
     spreader_.spread(pts_, grid_.get(), s);  // times ST_SPREAD (+ ST_MERGE) itself
+    if (fz_) {
+      // x and y passes per plane, then the z pass, the loss and the inverse
+      // z pass of the pullback per tile of band columns
+      {
+        ProfScope p(ST_FFT_FWD, s);
+        bdft_.forward_planes(grid_.get(), s);
+      }
+      ProfScope p(ST_LOSS, s);
+      launch_zloss(1, s, e_out, e_host);
+      return;
+    }
     if (bd_) {
       // the band transform's last pass also clears the pullback spectrum
       ProfScope p(ST_FFT_FWD, s);
@@ -443,7 +421,9 @@ class NUwU final : public Potential {
     } else {
       {
         ProfScope p(ST_FFT_INV, s);
-        if (bd_)
+        if (fz_)
+          bdft_.inverse_planes(grid_.get(), s);
+        else if (bd_)
           bdft_.inverse(X_[0].get(), grid_.get(), s);
         else
           c2r_.exec(X_[0].get(), grid_.get(), s);
@@ -497,7 +477,9 @@ class NUwU final : public Potential {
     if (bd_) {
       bdft_.init(geom_.nf, band);
       nspec_ = bdft_.size();
+      fz_ = bdft_.fused();
     } else {
+      fz_ = false;
       nspec_ = (int64_t)hx_ * geom_.nf[1] * geom_.nf[2];
     }
     S_.allocate(nspec_);
@@ -525,6 +507,11 @@ class NUwU final : public Potential {
     hipLaunchKernelGGL((k_nuwu_loss<T, MODE>), dim3(nblk_), dim3(256), 0, s, S_.get(), pd_, npairs_,
                        (double)N_, 4.0 / (double)N_, X0, X1, X2, d_, partial_.get(), ticket_.get(), e_out, e_host);
     FRESCO_HIP_CHECK(hipGetLastError());
+  }
+
+  void launch_zloss(int mode, hipStream_t s, double* e_out, double* e_host) {
+    bdft_.zloss(mode, pd_, bl_, (double)N_, 4.0 / (double)N_, S_.get(), partial_.get(), ticket_.get(), e_out, e_host,
+                s);
   }
 
   void build_pairs(const double* Sk, const double* V) {
@@ -567,6 +554,58 @@ class NUwU final : public Potential {
       ip.push_back(1.0 / ps);
     }
     npairs_ = (int)h.size();
+    std::vector<int> loc, mir, pstart;
+    if (fz_) {
+      // fused band form: pairs grouped by the z-pass tile of their
+      // representative's column (stable), with their places in the tile
+      const int B = bdft_.band();
+      std::vector<int> tile(npairs_);
+      loc.resize(npairs_);
+      mir.resize(npairs_);
+      for (int q = 0; q < npairs_; ++q) {
+        const int k0 = kv[3 * q], k1 = kv[3 * q + 1], k2 = kv[3 * q + 2];
+        int t, sl;
+        bdft_.column_slot(k1, k0, &t, &sl);
+        tile[q] = t;
+        loc[q] = (k2 + B) * 16 + sl;
+        mir[q] = -1;
+        if (hn[q] >= 0) {
+          int t2, sl2;
+          bdft_.column_slot(-k1, 0, &t2, &sl2);
+          if (t2 != t) throw RuntimeFailure("band transform: a column and its mirror in different tiles");
+          mir[q] = (-k2 + B) * 16 + sl2;
+        }
+      }
+      std::vector<int> order(npairs_);
+      for (int q = 0; q < npairs_; ++q) order[q] = q;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tile[a] < tile[b]; });
+      auto perm_i = [&](std::vector<int>& v, int per) {
+        std::vector<int> o((size_t)npairs_ * per);
+        for (int q = 0; q < npairs_; ++q)
+          for (int c = 0; c < per; ++c) o[(size_t)q * per + c] = v[(size_t)order[q] * per + c];
+        v.swap(o);
+      };
+      auto perm_d = [&](std::vector<double>& v) {
+        std::vector<double> o(npairs_);
+        for (int q = 0; q < npairs_; ++q) o[q] = v[order[q]];
+        v.swap(o);
+      };
+      perm_i(h, 1);
+      perm_i(hn, 1);
+      perm_i(ik, 1);
+      perm_i(imk, 1);
+      perm_i(kv, 3);
+      perm_i(loc, 1);
+      perm_i(mir, 1);
+      perm_d(vk);
+      perm_d(vm);
+      perm_d(sk);
+      perm_d(sm);
+      perm_d(ip);
+      pstart.assign(bdft_.ntiles() + 1, 0);
+      for (int q = 0; q < npairs_; ++q) ++pstart[tile[order[q]] + 1];
+      for (int t = 0; t < bdft_.ntiles(); ++t) pstart[t + 1] += pstart[t];
+    }
     std::vector<unsigned char> act(M_, 0);
     for (int q = 0; q < npairs_; ++q) act[ik[q]] = act[imk[q]] = 1;
     active_.allocate(M_);
@@ -595,8 +634,14 @@ class NUwU final : public Potential {
     up_d(p_ip_, ip);
     pd_ = PairDev{p_h_.get(), p_hn_.get(), p_ik_.get(), p_imk_.get(), p_kv_.get(), p_vk_.get(),
                   p_vm_.get(), p_sk_.get(), p_sm_.get(), p_ip_.get()};
+    if (fz_) {
+      up_i(p_loc_, loc, np);
+      up_i(p_mir_, mir, np);
+      up_i(p_pstart_, pstart, pstart.size());
+      bl_ = BandLossDev{p_pstart_.get(), p_loc_.get(), p_mir_.get()};
+    }
     nblk_ = std::max(1, (npairs_ + 255) / 256);
-    partial_.allocate(nblk_);
+    partial_.allocate(std::max(nblk_, fz_ ? bdft_.ntiles() : 1));
     if (!ticket_.get()) {
       ticket_.allocate(1);
       FRESCO_HIP_CHECK(hipMemset(ticket_.get(), 0, sizeof(unsigned)));
@@ -623,12 +668,15 @@ class NUwU final : public Potential {
   int nX_ = 1;
   FftPlan r2c_, c2r_;
   bool bd_ = false;  // spectrum through bdft_ (compact band layout)
+  bool fz_ = false;  // bdft_'s fused form (transforms and loss in three launches)
   BandDft<T> bdft_;
   std::vector<double> psi_host_[3];
   DeviceBuffer<double> psi_[3];
   DeviceBuffer<int> p_h_, p_hn_, p_ik_, p_imk_, p_kv_;
   DeviceBuffer<double> p_vk_, p_vm_, p_sk_, p_sm_, p_ip_;
   PairDev pd_{};
+  DeviceBuffer<int> p_loc_, p_mir_, p_pstart_;  // pair tiles of the fused band form
+  BandLossDev bl_{};
   int npairs_ = 0, nblk_ = 1;
   DeviceBuffer<double> partial_, e_dev_;
   DeviceBuffer<unsigned> ticket_;  // last-workgroup ticket of the loss kernel

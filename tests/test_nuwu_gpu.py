@@ -359,8 +359,11 @@ def test_direct_binning_equals_radix_sort(NUwU, d, n, N, monkeypatch):
 @pytest.mark.parametrize("kmax", [6, 12])
 def test_band_dft_matches_fft_path(NUwU, kmax, monkeypatch):
     """3D FP32 adjoint path with an active band |k_a| <= B small against the
-    fine grid: its spectrum comes from pruned separable DFTs of the band (six
-    twiddle-matrix passes) instead of the grid's real FFTs.  Same energy,
+    fine grid: its spectrum comes from pruned separable DFTs of the band
+    instead of the grid's real FFTs, in the fused form (x and y passes per
+    plane, the z pass with the loss and the inverse z pass per tile of band
+    columns, y and x passes per plane) and in the four-launch form
+    (FRESCO_BANDDFT_FUSED=0).  Same energy (with and without the gradient),
     gradient and target derivatives as the FFT path (FRESCO_BANDDFT=0) to
     FP32 rounding, and parity with the exact sums."""
     rng = np.random.default_rng(50 + kmax)
@@ -372,16 +375,21 @@ def test_band_dft_matches_fft_path(NUwU, kmax, monkeypatch):
     radii = rng.uniform(0.7, 1.3, N)
     p = rng.uniform(0, 1, 3 * N)
     out = {}
-    for mode in ("1", "0"):
+    for mode, fused in (("1", "1"), ("1", "0"), ("0", "1")):
         monkeypatch.setenv("FRESCO_BANDDFT", mode)
+        monkeypatch.setenv("FRESCO_BANDDFT_FUSED", fused)
         pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
         E, g = pot.get_energy_gradient(p)
+        Ee = pot.get_energy(p)
         dS0, dV = pot.target_grads()
-        out[mode] = (E, g, dS0.cpu().numpy(), dV.cpu().numpy())
-    (E1, g1, s1, v1), (E0, g0, s0, v0) = out["1"], out["0"]
-    assert abs(E1 - E0) / abs(E0) < 2e-6
-    assert rel(g1, g0) < 2e-6
-    assert rel(s1, s0) < 2e-6 and rel(v1, v0) < 2e-6
+        out[mode + fused] = (E, g, dS0.cpu().numpy(), dV.cpu().numpy(), Ee)
+    (E1, g1, s1, v1, e1) = out["11"]
+    for key in ("10", "01"):
+        E0, g0, s0, v0, e0 = out[key]
+        assert abs(E1 - E0) / abs(E0) < 2e-6, key
+        assert abs(e1 - e0) / abs(E0) < 2e-6, key
+        assert rel(g1, g0) < 2e-6, key
+        assert rel(s1, s0) < 2e-6 and rel(v1, v0) < 2e-6, key
     ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
     Er, gr = ref.get_energy_gradient(p)
     assert abs(E1 - Er) / abs(Er) < 1e-5
@@ -403,19 +411,21 @@ def test_band_dft_smallest_bands(NUwU, band, monkeypatch):
     radii = rng.uniform(0.8, 1.2, N)
     p = rng.uniform(0, 1, 3 * N)
     res = {}
-    for mode in ("1", "0"):
+    for mode, fused in (("1", "1"), ("1", "0"), ("0", "1")):
         monkeypatch.setenv("FRESCO_BANDDFT", mode)
+        monkeypatch.setenv("FRESCO_BANDDFT_FUSED", fused)
         pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
-        res[mode] = pot.get_energy_gradient(p)
-    E, g = res["1"]
-    if band is None:
-        assert E == 0.0 and not np.any(g)
-        return
-    Er, gr = O.NUwU(radii, 0.0, Sk, V, np.ones(3)).get_energy_gradient(p)
-    assert abs(E - Er) / abs(Er) < 1e-5
-    assert abs(E - res["0"][0]) / abs(Er) < 2e-6
-    gn = max(np.linalg.norm(gr), 1e-300)
-    assert np.linalg.norm(g - gr) / gn < 1e-5 or np.linalg.norm(g - gr) < 1e-5 * abs(Er)
+        res[mode + fused] = pot.get_energy_gradient(p)
+    for key in ("11", "10"):
+        E, g = res[key]
+        if band is None:
+            assert E == 0.0 and not np.any(g)
+            continue
+        Er, gr = O.NUwU(radii, 0.0, Sk, V, np.ones(3)).get_energy_gradient(p)
+        assert abs(E - Er) / abs(Er) < 1e-5
+        assert abs(E - res["01"][0]) / abs(Er) < 2e-6
+        gn = max(np.linalg.norm(gr), 1e-300)
+        assert np.linalg.norm(g - gr) / gn < 1e-5 or np.linalg.norm(g - gr) < 1e-5 * abs(Er)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64-adjoint"])
