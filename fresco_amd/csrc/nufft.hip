@@ -541,6 +541,17 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   rec_in.allocate(n);
   key.allocate(n);
   // FRESCO_DIRECT_BINNING=0: always the radix sort (the same order)
+  // interpolation layout: FRESCO_INTERP_LPP=8 (eight lanes per point instead
+  // of four), FRESCO_INTERP_WGS (target workgroup count, default 2048),
+  // FRESCO_INTERP_UNITS=0 (row-wise region staging instead of 16-byte units)
+  {
+    const char* e = std::getenv("FRESCO_INTERP_LPP");
+    interp_lpp = e && std::atoi(e) == 8 ? 8 : 4;
+    e = std::getenv("FRESCO_INTERP_WGS");
+    interp_wgs = e ? std::max(1, std::atoi(e)) : 2048;
+    e = std::getenv("FRESCO_INTERP_UNITS");
+    interp_units = !e || std::atoi(e) != 0;
+  }
   const char* env = std::getenv("FRESCO_DIRECT_BINNING");
   direct = tb <= PB_MAXB && !(env && env[0] == '0');
   if (direct) {
@@ -2874,6 +2885,41 @@ static void launch_interp3_lane(const SortedPoints<T>& p, const T* grid, const d
 constexpr int G8_THREADS = 256;
 constexpr int G8_SORT_MAX = 1024;  // slice points reordered in LDS
 
+// out[j] = value j of the quad (lane j % 4 holds values j % 4 and j % 4 + 4
+// in v2[0], v2[1]): DPP quad_perm broadcasts
+// (no lane of a quad_perm is out of range: the move needs no old value)
+template <int J>
+__device__ __forceinline__ float quad_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J | (J << 2) | (J << 4) | (J << 6), 0xf, 0xf, true));
+}
+template <int J>
+__device__ __forceinline__ double quad_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  constexpr int C = J | (J << 2) | (J << 4) | (J << 6);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), C, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), C, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ void quad_bcast8(const T* v2, T* out) {
+  out[0] = quad_lane<0>(v2[0]);
+  out[1] = quad_lane<1>(v2[0]);
+  out[2] = quad_lane<2>(v2[0]);
+  out[3] = quad_lane<3>(v2[0]);
+  out[4] = quad_lane<0>(v2[1]);
+  out[5] = quad_lane<1>(v2[1]);
+  out[6] = quad_lane<2>(v2[1]);
+  out[7] = quad_lane<3>(v2[1]);
+}
+
+// the sum of a quad's values, complete in its last lane (and every lane)
+template <typename T>
+__device__ __forceinline__ T sum4_last(T v) {
+  v += dpp_f<0xb1>(v);  // quad_perm [1,0,3,2]
+  v += dpp_f<0x4e>(v);  // quad_perm [2,3,0,1]
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T sum8_last(T v) {
   v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
@@ -2886,7 +2932,7 @@ __device__ __forceinline__ T sum8_last(T v) {
 // time (the 16 x 8 x 8 and 16 x 16 x 8 bins), so the 64 cell reads of a
 // point become immediate offsets of one address (paired LDS reads, no
 // per-read address arithmetic); 0: taken from a
-template <typename T, bool DERIV, int SRX = 0, int RY = 0>
+template <typename T, bool DERIV, int SRX = 0, int RY = 0, int LPP = 8>
 __global__ void __launch_bounds__(G8_THREADS)
     k_interp3_g8(const uint32_t* __restrict__ scell, const typename Vec4<T>::type* __restrict__ rec,
                  const uint32_t* __restrict__ idx, const int* __restrict__ bin_start, const T* __restrict__ grid,
@@ -2908,6 +2954,19 @@ __global__ void __launch_bounds__(G8_THREADS)
   t0[0] = (b % a.ntile[0]) * a.tile[0];
   t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
   t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  // the slice's packed first cells for the counting sort below, loaded
+  // together with the region (one memory latency for both)
+  const int npts = end - beg;
+  const bool sorted = npts <= G8_SORT_MAX;
+  __shared__ uint16_t perm[G8_SORT_MAX];
+  __shared__ int kcnt[128];
+  uint32_t skey[G8_SORT_MAX / G8_THREADS];
+#pragma unroll
+  for (int u = 0; u < G8_SORT_MAX / G8_THREADS; ++u) {
+    const int e = tid + u * G8_THREADS;
+    skey[u] = sorted && e < npts ? scell[beg + e] : 0u;
+  }
+  if (tid < 128) kcnt[tid] = 0;
   // region rows of up to 32 cells: thread (row slot, x)
   {
     const int lx = tid & 31;
@@ -2982,11 +3041,10 @@ __global__ void __launch_bounds__(G8_THREADS)
     }
   }
   __syncthreads();
-  const int ja = tid & 7;
-  const int grp = tid >> 3;
+  const int ja = tid & (LPP - 1);
+  const int grp = tid / LPP;
   const int srx = SRX ? SRX : a.srx;
   const int zs = (RY ? RY : a.ry) * srx;
-  const int npts = end - beg;
   // The 8 points one wavefront interpolates together read 8-cell windows of
   // the region; at unrelated (y, z) rows those windows collide in the LDS
   // banks.  Points taken in the order of their first (z, y) cell share their
@@ -2994,19 +3052,14 @@ __global__ void __launch_bounds__(G8_THREADS)
   // same words.  Each point's result does not depend on the order, so the
   // slice is reordered by a counting sort in LDS (slices up to G8_SORT_MAX
   // points; longer ones keep the bin order).
-  __shared__ uint16_t perm[G8_SORT_MAX];
-  __shared__ int kcnt[128];
-  const bool sorted = npts <= G8_SORT_MAX;
   if (sorted) {
-    if (tid < 128) kcnt[tid] = 0;
-    __syncthreads();
     int key[G8_SORT_MAX / G8_THREADS];
 #pragma unroll
     for (int u = 0; u < G8_SORT_MAX / G8_THREADS; ++u) {
       const int e = tid + u * G8_THREADS;
       key[u] = -1;
       if (e < npts) {
-        const uint32_t sc = scell[beg + e];
+        const uint32_t sc = skey[u];
         const int uy = (int)((sc >> a.cbits) & cmask) - t0[1];
         const int uz = (int)((sc >> (2 * a.cbits)) & cmask) - t0[2];
         key[u] = ((uz & 7) << 4) | (uy & 15);
@@ -3035,7 +3088,7 @@ __global__ void __launch_bounds__(G8_THREADS)
   }
   auto point_at = [&](int pos) { return pos < npts ? beg + (sorted ? (int)perm[pos] : pos) : beg; };
   // every group runs the same number of iterations (DPP needs all lanes)
-  const int iters = (npts + (G8_THREADS / 8) - 1) / (G8_THREADS / 8);
+  const int iters = (npts + (G8_THREADS / LPP) - 1) / (G8_THREADS / LPP);
   // the next iteration's point is loaded while this one is interpolated
   int pn = grp;
   int kq0 = point_at(pn);
@@ -3047,7 +3100,7 @@ __global__ void __launch_bounds__(G8_THREADS)
     const uint32_t sc = sc_n;
     const typename Vec4<T>::type r = r_n;
     const uint32_t jout = j_n;
-    pn += G8_THREADS / 8;
+    pn += G8_THREADS / LPP;
     if (it + 1 < iters) {
       const int kq = point_at(pn);
       sc_n = scell[kq];
@@ -3061,6 +3114,78 @@ __global__ void __launch_bounds__(G8_THREADS)
       if (u < 0) u += a.nf[q];
       l0[q] = u;
     }
+    if constexpr (LPP == 4) {
+      // four lanes per point: lane ja takes the x offsets ja and ja + 4 and
+      // evaluates (phi, phi') of indices ja and ja + 4 along x, y and z; the
+      // quad shares the y and z values by DPP quad broadcasts and reduces
+      // with two DPP steps
+      T wx2[2] = {T(0), T(0)}, dwx2[2] = {T(0), T(0)}, wy2[2] = {T(0), T(0)}, dwy2[2] = {T(0), T(0)};
+      T wz2[2] = {T(0), T(0)}, dwz2[2] = {T(0), T(0)};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int jj = ja + 4 * h;
+        if (jj < w) {
+          if (DERIV) {
+            es_phi_dphi<T>((T(jj) - r.x) * a.k.two_over_w, a.k, wx2[h], dwx2[h]);
+            es_phi_dphi<T>((T(jj) - r.y) * a.k.two_over_w, a.k, wy2[h], dwy2[h]);
+            es_phi_dphi<T>((T(jj) - r.z) * a.k.two_over_w, a.k, wz2[h], dwz2[h]);
+          } else {
+            wx2[h] = es_phi<T>((T(jj) - r.x) * a.k.two_over_w, a.k);
+            wy2[h] = es_phi<T>((T(jj) - r.y) * a.k.two_over_w, a.k);
+            wz2[h] = es_phi<T>((T(jj) - r.z) * a.k.two_over_w, a.k);
+          }
+        }
+      }
+      T wy[8], dwy[8], wz[8], dwz[8];
+      quad_bcast8<T>(wy2, wy);
+      quad_bcast8<T>(wz2, wz);
+      if (DERIV) {
+        quad_bcast8<T>(dwy2, dwy);
+        quad_bcast8<T>(dwz2, dwz);
+      }
+      const T* col = reg + (l0[2] * (RY ? RY : a.ry) + l0[1]) * srx + l0[0] + ja;
+      T sA[2] = {T(0), T(0)}, sB[2] = {T(0), T(0)}, sC[2] = {T(0), T(0)};
+#pragma unroll
+      for (int jc = 0; jc < 8; ++jc) {
+        T pA[2] = {T(0), T(0)}, pB[2] = {T(0), T(0)};
+#pragma unroll
+        for (int jb = 0; jb < 8; ++jb) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const T g = col[jc * zs + jb * srx + 4 * h];
+            pA[h] += g * wy[jb];
+            if (DERIV) pB[h] += g * dwy[jb];
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          sA[h] += pA[h] * wz[jc];
+          if (DERIV) {
+            sB[h] += pB[h] * wz[jc];
+            sC[h] += pA[h] * dwz[jc];
+          }
+        }
+      }
+      T vx, vy = T(0), vz = T(0);
+      if (DERIV) {
+        vx = sum4_last(dwx2[0] * sA[0] + dwx2[1] * sA[1]);
+        vy = sum4_last(wx2[0] * sB[0] + wx2[1] * sB[1]);
+        vz = sum4_last(wx2[0] * sC[0] + wx2[1] * sC[1]);
+      } else {
+        vx = sum4_last(wx2[0] * sA[0] + wx2[1] * sA[1]);
+      }
+      if (ja == 3 && valid) {
+        const int64_t j = jout;
+        const double cw = (double)r.w;
+        if (DERIV) {
+          out[j * a.ostride + 0] = cw * a.dscale[0] * (double)vx;
+          out[j * a.ostride + 1] = cw * a.dscale[1] * (double)vy;
+          out[j * a.ostride + 2] = cw * a.dscale[2] * (double)vz;
+        } else {
+          out[j * a.ostride + a.ocomp] = cw * (double)vx;
+        }
+      }
+    } else {
     // lane ja evaluates (phi, phi') of index ja along x, y and z; the group
     // shares the y and z values by ds_swizzle (broadcast of lane j of the
     // 8-lane group)
@@ -3121,6 +3246,7 @@ __global__ void __launch_bounds__(G8_THREADS)
         out[j * a.ostride + a.ocomp] = cw * (double)vx;
       }
     }
+    }
   }
 }
 
@@ -3148,25 +3274,28 @@ static void launch_interp3_g8(const SortedPoints<T>& p, const T* grid, const dou
   a.k.two_over_w = (T)(2.0 / p.es.w);
   size_t lds = sizeof(T) * (size_t)a.rx * a.ry * a.rz;
   if (lds > 160 * 1024) throw InvalidArgument("interpolation region exceeds LDS");
-  // enough workgroups to fill the GPU: split bins when there are few
-  // ~4096 workgroups (4 per bin for the 1183 bins of a 100^3 grid; measured 0.132 / 0.118 /
-  // 0.112 / 0.112 ms for 1024 / 2048 / 3072 / 4096)
-  const int chunks = std::max(1, std::min(8, (4096 + g.nbins - 1) / g.nbins));
+  // enough workgroups to fill the GPU: bins split into slices when there
+  // are few, ~2048 workgroups (2 per bin for the 1183 bins of a 100^3 grid;
+  // four lanes per point: 0.128 / 0.080 / 0.081 / 0.088 / 0.096 ms for 1 /
+  // 2 / 3 / 4 / 6 slices per bin; FRESCO_INTERP_WGS overrides the target)
+  const int want = p.interp_wgs;
+  const int chunks = std::max(1, std::min(8, (want + g.nbins - 1) / g.nbins));
   const dim3 grd(g.nbins, chunks);
   // 16-byte row units: whole units per grid row, the tile's first cell on a
   // unit, one wrap per dimension at most, 32-bit offsets, an aligned grid
-  static const bool units_on = [] {
-    const char* e = std::getenv("FRESCO_INTERP_UNITS");
-    return !e || std::atoi(e) != 0;
-  }();
+  const bool units_on = p.interp_units;
   const int epu = 16 / (int)sizeof(T);
   const bool units = units_on && a.rx == 23 && a.rz == 15 && (a.ry == 15 || a.ry == 23) && g.nf[0] % epu == 0 &&
                      g.tile[0] % epu == 0 && g.nf[0] >= 24 && g.nf[1] >= a.ry && g.nf[2] >= 15 &&
                      g.ncells() < (1ll << 31) && ((uintptr_t)grid % 16) == 0;
+  const int lpp = p.interp_lpp;
   if (units) {
     lds = sizeof(T) * (size_t)24 * a.ry * a.rz;
     a.srx = 24;
-    if (a.ry == 15)
+    if (a.ry == 15 && lpp == 4)
+      hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 24, 15, 4>), grd, dim3(G8_THREADS), lds, s, p.scell.get(),
+                         p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
+    else if (a.ry == 15)
       hipLaunchKernelGGL((k_interp3_g8<T, DERIV, 24, 15>), grd, dim3(G8_THREADS), lds, s, p.scell.get(),
                          p.rec.get(), p.idx.get(), p.bin_start.get(), grid, a, out);
     else

@@ -93,6 +93,8 @@ class SortedPoints {
   // direct: one counting pass over tiles (all bins of the plan fit one
   // 11-bit digit, no segments); else the radix sort over the keys
   bool direct = false;
+  int interp_lpp = 4, interp_wgs = 2048;  // 3D interpolation layout (set at init)
+  bool interp_units = true;
   BinSort sorter;                                       // stable LSD radix sort (sort.hip)
   DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point

@@ -303,6 +303,37 @@ def test_matrix_core_spreader(NUwU, case, monkeypatch):
 
 
 @pytest.mark.parametrize("case", ["uniform", "clustered"])
+def test_interpolation_lane_layouts_agree(NUwU, case, monkeypatch):
+    """3D FP32 gradient interpolation from 16-byte staged regions with four
+    lanes per point (two x offsets each, quad broadcasts and reductions) and
+    with eight (FRESCO_INTERP_LPP=8), at one and two slices per bin: parity
+    with the exact sums, agreement to FP32 rounding, bitwise repeats.  The
+    clustered case gives slices of more than 1024 points (interpolated in
+    bin order, not sorted)."""
+    n, N = 33, 6000
+    radii, Sk, V, p = make_case(3, n, N, seed=83)
+    if case == "clustered":
+        p = _clustered(N, 0.7, seed=84)
+    ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
+    E0, g0 = ref.get_energy_gradient(p)
+    res = {}
+    for lpp, wgs in (("4", "2048"), ("4", "1"), ("8", "2048")):
+        monkeypatch.setenv("FRESCO_INTERP_LPP", lpp)
+        monkeypatch.setenv("FRESCO_INTERP_WGS", wgs)
+        pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        info = pot.plan_info()
+        assert info["nf"][0] % 4 == 0 and info["tile"] == [16, 8, 8], info
+        E, g = pot.get_energy_gradient(p)
+        assert abs(E - E0) / abs(E0) < 1e-5
+        assert rel(g, g0) < 1e-5
+        E2, g2 = pot.get_energy_gradient(p)
+        assert E == E2 and np.array_equal(g, g2)
+        res[lpp + ":" + wgs] = g
+    assert rel(res["4:2048"], res["8:2048"]) < 2e-6
+    assert rel(res["4:1"], res["4:2048"]) < 2e-6
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered"])
 def test_matrix_core_spreader_2d(NUwU, case, monkeypatch):
     """2D FP32 halo spreading as matrix products (16 x 16 bins: four 16 x 16
     windows of the 23 x 23 halo tile, one product per 4 points of a window,
