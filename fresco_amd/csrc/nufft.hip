@@ -317,7 +317,7 @@ __global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m,
 // the radix sort gives, without its extra key and index arrays.
 // ---------------------------------------------------------------------------
 constexpr int PB_T = 256;
-constexpr int PB_ROUNDS = 8;
+constexpr int PB_ROUNDS = 4;  // 1024-point tiles: placement 27 -> 24 us at N = 2^20 (2048-point tiles)
 constexpr int PB_TILE = PB_T * PB_ROUNDS;
 constexpr int PB_MAXB = 2048;
 constexpr int PB_GMAX = 32;  // at most this many tile groups
