@@ -709,8 +709,7 @@ __global__ void __launch_bounds__(BZ_T)
     k_band_zloss(const Cplx<float>* __restrict__ U2, BandShape sh, const int* __restrict__ tcol,
                  const Cplx<float>* __restrict__ Wz, const Cplx<float>* __restrict__ Wzip, PairDev pd,
                  BandLossDev bl, double norm, double gscale, Cplx<float>* __restrict__ S,
-                 Cplx<float>* __restrict__ Y1, double* __restrict__ partial, unsigned* __restrict__ ticket,
-                 double* __restrict__ e_out, double* __restrict__ e_host) {
+                 Cplx<float>* __restrict__ Y1, double* __restrict__ partial) {
   __shared__ __align__(16) Cplx<float> ut[128][BZ_COLS];          // U2 columns [z][slot]
   __shared__ __align__(16) Cplx<float> sx[2][BZ_KROWS][BZ_COLS];  // S and X [kz][slot]
   extern __shared__ __align__(16) unsigned char smem[];           // Wzi [nzR][SWZI] (MODE 1)
@@ -834,16 +833,18 @@ __global__ void __launch_bounds__(BZ_T)
       }
     }
   }
-  // the last workgroup sums the tiles' energies in their fixed order
-  if (last_block_arrived(ticket, gridDim.x)) block_sum_partials<BZ_T>(partial, gridDim.x, e_out, e_host);
 }
 
 // Y1 [nz][nk][nb], Wyi padded [nyR][SWYI], Wxi padded [nxR][SWXI] (kx > 0
 // weighted twice) -> grid
 __global__ void __launch_bounds__(BP_T)
     k_band_plane_inv(const Cplx<float>* __restrict__ Y1, BandShape sh, const Cplx<float>* __restrict__ Wyip,
-                     const Cplx<float>* __restrict__ Wxip, float* __restrict__ grid) {
+                     const Cplx<float>* __restrict__ Wxip, float* __restrict__ grid, const double* __restrict__ partial,
+                     int nparts, double* __restrict__ e_out, double* __restrict__ e_host) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // the z pass's per-tile energies, summed by the first workgroup (the
+  // previous launch has completed, so they are visible without a fence)
+  if (partial && blockIdx.x == 0 && blockIdx.y == 0) block_sum_partials(partial, nparts, e_out, e_host);
   // workgroup (z, g): plane z, y tiles BP_YG g .. BP_YG g + BP_YG - 1 (both
   // passes keep y as the row index, so the groups are independent)
   const int z = blockIdx.x;
@@ -994,29 +995,30 @@ void BandDft<T>::forward_planes(const T* grid, hipStream_t s) {
 
 template <typename T>
 void BandDft<T>::zloss(int mode, const PairDev& pd, const BandLossDev& bl, double norm, double gscale, Cplx<T>* S,
-                       double* partial, unsigned* ticket, double* e_out, double* e_host, hipStream_t s) {
+                       double* partial, double* e_out, double* e_host, bool defer, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, 2 * B_ + 1);
     if (mode == 0) {
       hipLaunchKernelGGL(k_band_zloss<0>, dim3(ntiles_), dim3(BZ_T), 0, s, u2_.get(), sh, tcol_.get(), wc_.get(),
-                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial, ticket, e_out, e_host);
+                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial);
     } else {
       const size_t lds = sizeof(Cplx<float>) * (size_t)sh.nzR * sh.SWZI;
       hipLaunchKernelGGL(k_band_zloss<1>, dim3(ntiles_), dim3(BZ_T), lds, s, u2_.get(), sh, tcol_.get(), wc_.get(),
-                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial, ticket, e_out, e_host);
+                         wzip_.get(), pd, bl, norm, gscale, S, u1_.get(), partial);
     }
     FRESCO_HIP_CHECK(hipGetLastError());
+    if (!defer || mode == 0) reduce_partials(partial, ntiles_, e_out, s, e_host);
   } else {
     throw RuntimeFailure("band transform: the fused form is FP32 only");
   }
 }
 
 template <typename T>
-void BandDft<T>::inverse_planes(T* grid, hipStream_t s) {
+void BandDft<T>::inverse_planes(T* grid, hipStream_t s, const double* partial, double* e_out, double* e_host) {
   if constexpr (std::is_same<T, float>::value) {
     const BandShape sh(nf_[0], nf_[1], nf_[2], nb_, 2 * B_ + 1);
     hipLaunchKernelGGL(k_band_plane_inv, dim3(nf_[2], (sh.nyR / 16 + BP_YG - 1) / BP_YG), dim3(BP_T), sh.inv_lds(), s,
-                       u1_.get(), sh, wyip_.get(), wxip_.get(), grid);
+                       u1_.get(), sh, wyip_.get(), wxip_.get(), grid, partial, ntiles_, e_out, e_host);
     FRESCO_HIP_CHECK(hipGetLastError());
   } else {
     throw RuntimeFailure("band transform: the fused form is FP32 only");

@@ -77,9 +77,13 @@ class BandDft {
     *slot = col_pos_[c] % 16;
   }
   void forward_planes(const T* grid, hipStream_t s);
+  // The energy is summed from the tiles' partials (fixed order) by a
+  // one-workgroup launch after the loss, or, with defer, by the first
+  // workgroup of the next inverse_planes given the partials.
   void zloss(int mode, const PairDev& pd, const BandLossDev& bl, double norm, double gscale, Cplx<T>* S,
-             double* partial, unsigned* ticket, double* e_out, double* e_host, hipStream_t s);
-  void inverse_planes(T* grid, hipStream_t s);
+             double* partial, double* e_out, double* e_host, bool defer, hipStream_t s);
+  void inverse_planes(T* grid, hipStream_t s, const double* partial = nullptr, double* e_out = nullptr,
+                      double* e_host = nullptr);
 
  private:
   int nf_[3] = {0, 0, 0};

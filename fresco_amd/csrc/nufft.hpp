@@ -199,22 +199,25 @@ void fill_zero(T* p, size_t n, hipStream_t s);
 // and -> *out_host (device-visible pinned host memory) when not null.
 void reduce_partials(const double* partials, int n, double* out, hipStream_t s, double* out_host = nullptr);
 
-// The same compensated, fixed-order sum by one workgroup of NT threads
-// (every thread calls it; NT a power of two): part[n] -> *out, and
-// *out_host (device-visible pinned host memory) when not null.
+// The same compensated, fixed-order sum by the first NT threads of one
+// workgroup (every thread of it calls it, blockDim >= NT; NT a power of two):
+// part[n] -> *out, and *out_host (device-visible pinned host memory) when
+// not null.  The order depends on NT only, so every caller uses 256.
 template <int NT = 256>
 __device__ __forceinline__ void block_sum_partials(const double* part, int n, double* out, double* out_host) {
   __shared__ double ss[NT];
   __shared__ double cc[NT];
-  double sum = 0.0, comp = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) {
-    double y = part[i] - comp;
-    double t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
+  if ((int)threadIdx.x < NT) {
+    double sum = 0.0, comp = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+      double y = part[i] - comp;
+      double t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
+    }
+    ss[threadIdx.x] = sum;
+    cc[threadIdx.x] = comp;
   }
-  ss[threadIdx.x] = sum;
-  cc[threadIdx.x] = comp;
   __syncthreads();
   for (int h = NT / 2; h >= 1; h >>= 1) {
     if ((int)threadIdx.x < h) {

@@ -361,13 +361,23 @@ class NUwU final : public Potential {
   // e_host: device-visible pinned host memory that also receives the energy
   // (the host-synchronous entry points), or null (device-only: graphs)
   void enqueue_eg(const double* x, double* grad, double* e_out, double* e_host, hipStream_t s) {
+    if (fz_) {
+      // the tiles' energies are summed by the inverse transform's first
+      // workgroup (the launch after the loss: no fence, no extra launch)
+      enqueue_front(x, e_out, e_host, s, true);
+      kept_ = false;
+      enqueue_back(grad, s, e_out, e_host);
+      return;
+    }
     enqueue_front(x, e_out, e_host, s);
     kept_ = false;
     enqueue_back(grad, s);
   }
 
   // binning, spreading, forward transform, loss and pullback spectrum
-  void enqueue_front(const double* x, double* e_out, double* e_host, hipStream_t s) {
+  // defer (fused band form): the energy is left as per-tile partials for
+  // enqueue_back to sum
+  void enqueue_front(const double* x, double* e_out, double* e_host, hipStream_t s, bool defer = false) {
     last_x_ = x;
     {
       ProfScope p(ST_SORT, s);
@@ -382,7 +392,7 @@ class NUwU final : public Potential {
         bdft_.forward_planes(grid_.get(), s);
       }
       ProfScope p(ST_LOSS, s);
-      launch_zloss(1, s, e_out, e_host);
+      launch_zloss(1, s, e_out, e_host, defer);
       return;
     }
     if (bd_) {
@@ -408,7 +418,8 @@ class NUwU final : public Potential {
 
   // inverse transform(s) of the pullback spectrum and the interpolation of
   // the gradient at the sorted points
-  void enqueue_back(double* grad, hipStream_t s) {
+  // e_out / e_host not null: the fused band form's deferred energy sum
+  void enqueue_back(double* grad, hipStream_t s, double* e_out = nullptr, double* e_host = nullptr) {
     if (plain_) {
       for (int a = 0; a < d_; ++a) {
         {
@@ -422,7 +433,7 @@ class NUwU final : public Potential {
       {
         ProfScope p(ST_FFT_INV, s);
         if (fz_)
-          bdft_.inverse_planes(grid_.get(), s);
+          bdft_.inverse_planes(grid_.get(), s, e_out ? partial_.get() : nullptr, e_out, e_host);
         else if (bd_)
           bdft_.inverse(X_[0].get(), grid_.get(), s);
         else
@@ -509,9 +520,8 @@ class NUwU final : public Potential {
     FRESCO_HIP_CHECK(hipGetLastError());
   }
 
-  void launch_zloss(int mode, hipStream_t s, double* e_out, double* e_host) {
-    bdft_.zloss(mode, pd_, bl_, (double)N_, 4.0 / (double)N_, S_.get(), partial_.get(), ticket_.get(), e_out, e_host,
-                s);
+  void launch_zloss(int mode, hipStream_t s, double* e_out, double* e_host, bool defer = false) {
+    bdft_.zloss(mode, pd_, bl_, (double)N_, 4.0 / (double)N_, S_.get(), partial_.get(), e_out, e_host, defer, s);
   }
 
   void build_pairs(const double* Sk, const double* V) {
