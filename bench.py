@@ -178,14 +178,14 @@ def main():
     # headline run
     dom_ms = live[0] / live[1]
     achieved = alg_bytes[dom] / (dom_ms * 1e-3) / 1e9
-    kernels = {"halom": "k_spread3_halo_mfma2", "halo4": "k_spread3_halo_mw", "halo": "k_spread3_halo",
-               "tiles": "k_spread3_reg"}
+    kernels = {"halom": "k_spread3_halo_mfma2", "halor": "k_spread3_halo_ring", "halo4": "k_spread3_halo_mw",
+               "halo": "k_spread3_halo", "tiles": "k_spread3_reg"}
     kname = kernels.get(info32["spreader"], "k_spread") if dom == "spread" else "k_interp3_g8"
     w = info32["w"]
     alg_flops = 2.0 * N * w ** d  # one multiply-add per support cell and point
     hbm_view = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": alg_bytes[dom]}
-    if dom == "spread" and info32["spreader"] == "halom":
+    if dom == "spread" and info32["spreader"] in ("halom", "halor"):
         tflops = alg_flops / (dom_ms * 1e-3) / 1e12
         roofline = {
             "kernel": kname,
@@ -286,6 +286,8 @@ def spreader_label(info):
     name, parts = info["spreader"], info["spread_parts"]
     if name == "halom":
         split = "%d slice(s) x 2 x windows, one halo tile per slice" % (parts // 2)
+    elif name == "halor":
+        split = "%d slice(s) x 2 x windows, one tile per window" % (parts // 2)
     else:
         split = "%d slice(s)" % parts
     return "%s:%d, %s (%s)" % (name, parts, split, info["spread_source"])

@@ -716,6 +716,9 @@ __device__ __forceinline__ void group8_bcast<double>(double v, double* out) {
 constexpr int HW = 8;
 constexpr int HS_X = 24;  // row stride: (ja, jb) lane pattern conflict-free for 4- and 8-byte cells
 constexpr int HP_Z = 8 + HW - 1;
+// window tile of the ring spreader (k_spread3_halo_ring): [15 rows][16
+// planes][16 columns] (plane 15 zero), two per bin slice
+constexpr int HR_WT = 15 * 16 * 16;
 template <int TY>
 struct HaloShape {
   static constexpr int HP_Y = TY + HW - 1;
@@ -1043,29 +1046,56 @@ void Spreader<T>::init(const GridGeom& g, const EsParams& e) {
       }
       tabs.insert(tabs.end(), t.begin(), t.end());
     }
+    {
+      // x windows of the ring spreader: window 2 b + h covers columns
+      // [16 b + 8 h, 16 b + 8 h + 16) and holds the points whose first x cell
+      // is in its first 8 columns
+      std::vector<int> t(4 * (size_t)g.nf[0], -1);
+      for (int b = 0; b < g.ntile[0]; ++b) {
+        const int tl = std::min(g.tile[0], g.nf[0] - b * g.tile[0]);
+        for (int hw = 0; hw < 2; ++hw) {
+          const int first = std::min(8, tl - 8 * hw);
+          if (first <= 0) continue;
+          const int ulim = std::min(15, first + e.w - 1);
+          for (int u = 0; u < ulim; ++u) {
+            const int c = (b * g.tile[0] + 8 * hw + u) % g.nf[0];
+            int k = 0;
+            while (k < 4 && t[4 * c + k] >= 0) ++k;
+            if (k == 4) throw RuntimeFailure("halo merge table overflow");
+            t[4 * c + k] = ((2 * b + hw) << 6) | u;
+          }
+        }
+      }
+      tabs.insert(tabs.end(), t.begin(), t.end());
+    }
     halo_tab.allocate(tabs.size());
     halo_tab.upload(tabs.data(), tabs.size());
     // slices per bin: more slices, more wavefronts in flight but more halo
     // tiles to write and merge; the autotuner picks among these
     max_halo_parts = std::max(1, std::min(16, (16384 + g.nbins - 1) / g.nbins));
     const int helems = g.tile[1] == 8 ? HaloShape<8>::ELEMS : HaloShape<16>::ELEMS;
-    halo_buf.allocate((size_t)g.nbins * max_halo_parts * helems);
+    // the ring spreader writes 2 window tiles per slice, at least one slice
+    const size_t per_bin =
+        std::max((size_t)max_halo_parts * helems, (size_t)std::max(2, max_halo_parts & ~1) * HR_WT);
+    halo_buf.allocate((size_t)g.nbins * per_bin);
     mode = HALO;
     halo_parts = std::min(4, max_halo_parts);
     tuned = false;
-    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] | halom[:<parts>] pins the
-    // choice (reproducible rounding across processes) instead of timing the
-    // candidates; halom (FP32 strengths, 16 x 8 x 8 bins; parts = 2 x windows
-    // per slice) falls back to halo4 otherwise
+    // FRESCO_SPREAD3=reg | halo[:<slices>] | halo4[:<slices>] | halom[:<parts>] |
+    // halor[:<parts>] pins the choice (reproducible rounding across processes)
+    // instead of timing the candidates; halom and halor (FP32 strengths,
+    // 16 x 8 x 8 bins; parts = 2 x windows per slice) fall back to halo4
+    // otherwise
     if (const char* env = std::getenv("FRESCO_SPREAD3")) {
       const std::string v(env);
       if (v == "reg") {
         mode = REG;
-      } else if (v.rfind("halom", 0) == 0) {
-        // halom:<parts>: two x windows per slice, so parts is even
-        mode = sizeof(T) == 4 && g.tile[1] == 8 ? HALOM : HALO4;
+      } else if (v.rfind("halom", 0) == 0 || v.rfind("halor", 0) == 0) {
+        // two x windows per slice, so parts is even
+        const bool mm = sizeof(T) == 4 && g.tile[0] == 16 && g.tile[1] == 8 && g.tile[2] == 8;
+        mode = !mm ? HALO4 : v[4] == 'm' ? HALOM : HALOR;
         if (v.size() > 6) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 6)));
-        if (mode == HALOM) halo_parts = std::max(2, std::min(max_halo_parts & ~1, (halo_parts + 1) & ~1));
+        if (mode != HALO4) halo_parts = std::max(2, std::min(max_halo_parts & ~1, (halo_parts + 1) & ~1));
       } else if (v.rfind("halo4", 0) == 0) {
         mode = HALO4;
         if (v.size() > 6) halo_parts = std::max(1, std::min(max_halo_parts, std::atoi(v.c_str() + 6)));
@@ -1165,8 +1195,9 @@ void Spreader<T>::autotune(const SortedPoints<T>& p, T* grid, hipStream_t s) con
     std::lock_guard<std::mutex> lock(g_tune_mu);
     int m = -1, pp = 0;
     if (tune_lookup(user_tune_path(), key, &m, &pp) || tune_lookup(shipped_tune_path(), key, &m, &pp)) {
-      const bool halo_mode =
-          m == HALO || m == HALO4 || (m == HALOM && sizeof(T) == 4 && geom.tile[1] == 8 && pp % 2 == 0);
+      const bool halo_mode = m == HALO || m == HALO4 ||
+                             ((m == HALOM || m == HALOR) && sizeof(T) == 4 && geom.tile[0] == 16 &&
+                              geom.tile[1] == 8 && geom.tile[2] == 8 && pp % 2 == 0);
       if ((m == REG && geom.tile[1] == 16) || (halo_mode && pp >= 1 && pp <= max_halo_parts)) {
         mode = m;
         halo_parts = pp;
@@ -1193,9 +1224,12 @@ void Spreader<T>::autotune_timed(const SortedPoints<T>& p, T* grid, hipStream_t 
     if (sl <= max_halo_parts) cands.push_back({HALO, sl});
   for (int sl : {1, 2, 3, 4})
     if (sl <= max_halo_parts) cands.push_back({HALO4, sl});
-  if (sizeof(T) == 4 && geom.tile[1] == 8)
+  if (sizeof(T) == 4 && geom.tile[0] == 16 && geom.tile[1] == 8 && geom.tile[2] == 8)
     for (int sl : {1, 2, 3})
-      if (2 * sl <= max_halo_parts) cands.push_back({HALOM, 2 * sl});
+      if (2 * sl <= max_halo_parts) {
+        cands.push_back({HALOM, 2 * sl});
+        cands.push_back({HALOR, 2 * sl});
+      }
   hipEvent_t e0, e1;
   FRESCO_HIP_CHECK(hipEventCreate(&e0));
   FRESCO_HIP_CHECK(hipEventCreate(&e1));
@@ -2028,6 +2062,12 @@ __device__ __forceinline__ void hm_static_for(F&& f) {
   }
 }
 
+// Order of a wavefront's own LDS accesses (they execute in program order):
+// keeps the compiler from moving LDS reads and writes across this point.  The
+// ranking of a window's points touches only that window's arrays, so its
+// steps need no workgroup barrier.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
 template <int UZ>
 __device__ __forceinline__ void hm_products(hm_f4 (&acc)[HP_Z], float av, float bx, const float4& zlo,
                                             const float4& zhi) {
@@ -2129,7 +2169,7 @@ __global__ void __launch_bounds__(128)
 #pragma unroll
     for (int u = 0; u < WPL; ++u)
       if (wkey[u] >= 0) atomicAdd(&wrun[wkey[u]], 1);
-    __syncthreads();
+    wave_lds_order();
     const int cnt = lane < HM_NK ? wrun[lane] : 0;
     const int pad = (cnt + HM_CH - 1) / HM_CH * HM_CH;
     int inc = pad;
@@ -2140,15 +2180,14 @@ __global__ void __launch_bounds__(128)
     }
     const int nslot = __builtin_amdgcn_readfirstlane(__shfl(inc, HM_NK - 1, 64));
     const int st = inc - pad;
-    __syncthreads();
+    wave_lds_order();
     if (lane < HM_NK) {
       wrun[lane] = st;
       wkend[lane] = st + pad;
     }
     for (int q = lane; q < nslot; q += 64) wperm[q] = 0xFFFF;
-    __syncthreads();
-    // stable placement, 64 points a round per wavefront (wn is uniform, so
-    // both wavefronts run the same rounds and barriers)
+    wave_lds_order();
+    // stable placement, 64 points a round per wavefront
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
       if (u * 64 >= wn) break;
@@ -2163,9 +2202,9 @@ __global__ void __launch_bounds__(128)
       }
       const int rank = __popcll(same & lt);
       if (valid) wperm[min(wrun[key] + rank, MAXSLOT - 1)] = (uint16_t)(u * 64 + lane);
-      __syncthreads();
+      wave_lds_order();
       if (valid && rank == 0) wrun[key] += __popcll(same);
-      __syncthreads();
+      wave_lds_order();
     }
     // slot -> point: the permutation entry of chunk c + 3 is read one chunk
     // before the record of that chunk, so neither waits on the other
@@ -2284,6 +2323,299 @@ __global__ void __launch_bounds__(128)
   }
 }
 
+// The matrix-core halo spreader with the y rows in a ring (FP32 strengths,
+// 16 x 8 x 8 bins): like k_spread3_halo_mfma2 a two-wavefront workgroup per
+// (bin, slice), wavefront h = x window h (columns [8 h, 8 h + 16)), but a
+// window's points are ranked by (uy, uz >> 1), the y and z of their first
+// cell, each run padded to whole chunks of 8 slots (empty slots weigh 0),
+// two groups of 4.  The 4 points of a group share uy, so their y supports
+// are the same 8 rows, and a product's 16 rows are 2 planes x those 8 rows:
+//   D_P[(dz, r)][j] += sum_p  wy_p[(r - uy) mod 8] wz_p[2P + dz - uz_p]  *  c_p wx_p[8 h + j - ux_p]
+// for the plane pairs P = uz >> 1 .. (uz >> 1) + 4 (5 products per group: a
+// point's 8 planes span 4 pairs, or 5 for odd uz).  Ring row r holds the tile
+// row y = r (mod 8) of [uy, uy + 8): once the runs of uy are done, row uy is
+// complete and is written out from the registers, and its ring row starts
+// over as row uy + 8.  Half of every product's multiply-adds carry points (a
+// quarter in the two-window spreader).  Each window writes its own tile
+// [15 rows][16 planes][16 columns]; the merge sums the windows of
+// neighbouring bins (stride 8 in x) like halo tiles.  Slices of more than WIN
+// points take rounds, the rounds after the first adding to the tile.
+// Ranking is stable (ballot ranks in index order): the summation order is
+// fixed and the result bitwise reproducible.
+constexpr int HR_NK = 32;  // keys: (uy, uz >> 1)
+constexpr int HR_CH = 8;   // slots per chunk: two groups of 4 (the K of a product), kernel values evaluated together
+constexpr int HR_RS = 48;  // floats per slot record: y[8] (ring order), x[16], z even [24..28], z odd [32..36]
+
+template <int P0>
+__device__ __forceinline__ void hr_products(hm_f4 (&acc)[8], float ay, float bx, const float4& z4, float z5) {
+  const float z[5] = {z4.x, z4.y, z4.z, z4.w, z5};
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    acc[P0 + i] = __builtin_amdgcn_mfma_f32_16x16x4f32(ay * z[i], bx, acc[P0 + i], 0, 0, 0);
+}
+
+// ring row R of every plane pair to tile row `row` (< 15), then cleared.
+// Row m = 8 dz + R of a product sits in register R % 4 of the lanes with
+// k4 = 2 dz + R / 4.  The row's 16 planes x 16 columns go through LDS (buf,
+// 256 floats) and out as one contiguous 1 KB store (whole cache lines).
+template <int R>
+__device__ __forceinline__ void hr_flush(hm_f4 (&acc)[8], float* __restrict__ out, float* __restrict__ buf, int row,
+                                         int lane, bool add) {
+  constexpr int REG = R & 3;
+  const int k4 = lane >> 4, i16 = lane & 15;
+  const bool mine = (k4 & 1) == (R >> 2);
+  // every lane writes the ring row, its own value or the one of lane l ^ 16
+  // (lanes l and l ^ 16 write the same value): no divergent branch around
+  // the accumulator reads
+  float* b = buf + (k4 >> 1) * 16 + i16;
+#pragma unroll
+  for (int P = 0; P < 8; ++P) {
+    const float own = acc[P][REG];
+    const float other = __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(own), 0x401F));
+    b[P * 32] = mine ? own : other;
+  }
+#pragma unroll
+  for (int P = 0; P < 8; ++P) acc[P][REG] = mine ? 0.f : acc[P][REG];
+  float4 v = reinterpret_cast<const float4*>(buf)[lane];
+  float4* o = reinterpret_cast<float4*>(out + row * 256) + lane;
+  if (add) {
+    const float4 q = *o;
+    v.x += q.x;
+    v.y += q.y;
+    v.z += q.z;
+    v.w += q.w;
+  }
+  *o = v;
+}
+
+template <int WIN>
+__global__ void __launch_bounds__(128)
+    k_spread3_halo_ring(const uint32_t* __restrict__ scell, const float4* __restrict__ rec,
+                        const int* __restrict__ bin_start, SpreadDev<float> a, float* __restrict__ halo) {
+  static_assert(WIN % 128 == 0, "a round is staged by 128 threads");
+  constexpr int MAXSLOT = WIN + (HR_CH - 1) * HR_NK;
+  constexpr int WPL = WIN / 64;   // round points per lane (ranking, per wavefront)
+  constexpr int SPL = WIN / 128;  // round points per thread (staging)
+  __shared__ __align__(16) float4 wrec[WIN];
+  __shared__ uint32_t wsc[WIN];
+  __shared__ uint16_t perm[2][MAXSLOT];
+  __shared__ __align__(16) float recs[2][2][HR_CH * HR_RS];  // [window][buffer]: kernel values of two chunks
+  __shared__ int kend[2][HR_NK];
+  __shared__ int run[2][HR_NK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int h = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the window tile base is scalar
+  const int b = blockIdx.x;
+  const int nsl = gridDim.y, sl = blockIdx.y;
+  const int beg0 = bin_start[b];
+  const int len = bin_start[b + 1] - beg0;
+  const int beg = beg0 + (int)(((int64_t)len * sl) / nsl);
+  const int end = beg0 + (int)(((int64_t)len * (sl + 1)) / nsl);
+  float* out = halo + ((int64_t)(b * nsl + sl) * 2 + h) * HR_WT;
+  if (beg == end) {
+    for (int e = lane; e < HR_WT; e += 64) out[e] = 0.f;
+    return;
+  }
+  const uint32_t cmask = (1u << a.cbits) - 1u;
+  int t0[3];
+  t0[0] = (b % a.ntile[0]) * a.tile[0];
+  t0[1] = ((b / a.ntile[0]) % a.ntile[1]) * a.tile[1];
+  t0[2] = (b / (a.ntile[0] * a.ntile[1])) * a.tile[2];
+  auto rel = [&](uint32_t sc, int q) {
+    int u = (int)((sc >> (q * a.cbits)) & cmask) - t0[q];
+    if (u < 0) u += a.nf[q];
+    return u;
+  };
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int kj = lane & 7;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint16_t* wperm = perm[h];
+  int* wrun = run[h];
+  int* wkend = kend[h];
+  hm_f4 acc[8];
+#pragma unroll
+  for (int P = 0; P < 8; ++P) acc[P] = hm_f4{0.f, 0.f, 0.f, 0.f};
+  for (int ws = beg; ws < end; ws += WIN) {
+    const int wn = min(WIN, end - ws);
+    // a later round adds to the tile the earlier ones wrote (the same lanes
+    // read back the addresses they stored: their stores complete first)
+    const bool add = ws != beg;
+    if (add) __threadfence();
+    // the thread index, opaque here: the staging and ranking addresses are
+    // formed in every round instead of being kept in registers across the
+    // products (a slice rarely takes more than one round)
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const int lq = tq & 63;
+    {
+      float4 wr[SPL];
+      uint32_t wk[SPL];
+#pragma unroll
+      for (int u = 0; u < SPL; ++u) {
+        const int e = u * 128 + tq;
+        wk[u] = e < wn ? scell[ws + e] : 0u;
+        wr[u] = e < wn ? rec[ws + e] : float4{0.f, 0.f, 0.f, 0.f};
+      }
+      __syncthreads();  // both windows are done with the previous round's records
+#pragma unroll
+      for (int u = 0; u < SPL; ++u) {
+        wrec[u * 128 + tq] = wr[u];
+        wsc[u * 128 + tq] = wk[u];
+      }
+    }
+    if (lane < HR_NK) wrun[lane] = 0;
+    __syncthreads();
+    // key (uy, uz >> 1) for the points of this wavefront's window; -1 others
+    int wkey[WPL];
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      const int e = u * 64 + lq;
+      const uint32_t sc = wsc[e];
+      const bool mine = e < wn && ((rel(sc, 0) >> 3) & 1) == h;
+      wkey[u] = mine ? ((rel(sc, 1) & 7) << 2) | ((rel(sc, 2) & 7) >> 1) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < WPL; ++u)
+      if (wkey[u] >= 0) atomicAdd(&wrun[wkey[u]], 1);
+    wave_lds_order();
+    const int cnt = lane < HR_NK ? wrun[lane] : 0;
+    const int pad = (cnt + HR_CH - 1) / HR_CH * HR_CH;
+    int inc = pad;
+#pragma unroll
+    for (int off = 1; off < HR_NK; off <<= 1) {
+      const int u = __builtin_amdgcn_ds_bpermute(max(lq - off, 0) << 2, inc);
+      if (lq >= off) inc += u;
+    }
+    const int nslot = __builtin_amdgcn_readlane(inc, HR_NK - 1);
+    const int st = inc - pad;
+    wave_lds_order();
+    if (lane < HR_NK) {
+      wrun[lane] = st;
+      wkend[lane] = st + pad;
+    }
+    for (int q = lq; q < nslot; q += 64) wperm[q] = 0xFFFF;
+    wave_lds_order();
+    // stable placement, 64 points a round per wavefront
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+      if (u * 64 >= wn) break;
+      const int key = wkey[u];
+      const bool valid = key >= 0;
+      uint64_t same = __ballot(valid);
+#pragma unroll
+      for (int bt = 0; bt < 5; ++bt) {
+        const bool bit = (key >> bt) & 1;
+        const uint64_t mk = __ballot(bit);
+        same &= bit ? mk : ~mk;
+      }
+      const int rank = __popcll(same & lt);
+      if (valid) wperm[min(wrun[key] + rank, MAXSLOT - 1)] = (uint16_t)(u * 64 + lq);
+      wave_lds_order();
+      if (valid && rank == 0) wrun[key] += __popcll(same);
+      wave_lds_order();
+    }
+    // slot -> point: the permutation entry of chunk c + 3 is read one chunk
+    // before the record of that chunk, so neither waits on the other
+    uint32_t scn;
+    float4 rcn;
+    bool emp;
+    int pslot, praw;
+    auto fetch = [&](int c0) {
+      pslot = c0 + (lane >> 3);
+      praw = (int)wperm[min(pslot, MAXSLOT - 1)];
+    };
+    // (nothing here waits for the loads: values() uses them a chunk later)
+    auto load = [&]() {
+      emp = pslot >= nslot || praw >= wn;
+      const int pq = emp ? 0 : praw;
+      scn = wsc[pq];
+      rcn = wrec[pq];
+    };
+    // lane (slot, kj): the slot's y value kj at its ring row, x value kj at
+    // its window column, z value kj at position kj + (uz & 1) of the parity
+    // split z list (positions 0..9, the two not written are zero)
+    auto values = [&](float* rb) {
+      float* rr = rb + (lane >> 3) * HR_RS;
+      const float4 r = rcn;
+      const bool empty = emp;
+      const int dx = rel(scn, 0) & 7;
+      const int uy = rel(scn, 1) & 7;
+      const int par = rel(scn, 2) & 1;
+      const bool on = kj < a.w;
+      float wx = es_phi<float>((float(kj) - r.x) * a.k.two_over_w, a.k);
+      float wy = es_phi<float>((float(kj) - r.y) * a.k.two_over_w, a.k);
+      float wz = es_phi<float>((float(kj) - r.z) * a.k.two_over_w, a.k);
+      wx = on && !empty ? r.w * wx : 0.f;
+      wy = on ? wy : 0.f;
+      wz = on ? wz : 0.f;
+      rr[(kj + uy) & 7] = wy;
+      rr[8 + kj + dx] = wx;
+      rr[8 + ((kj + dx + 8) & 15)] = 0.f;
+      const int pos = kj + par;
+      rr[24 + 8 * (pos & 1) + (pos >> 1)] = wz;
+      // the two zeros: position 9 (lane 0) and position 0 (odd uz) or 8 (even
+      // uz) (lane 1); the other lanes write to the record's unused tail
+      rr[kj == 0 ? 36 : kj == 1 ? (par ? 24 : 28) : 40 + kj] = 0.f;
+    };
+    struct Ops {
+      float ay, bx;
+      float4 z4;
+      float z5;
+    };
+    auto rbuf = [&](int i) { return &recs[0][0][0] + (2 * h + i) * (HR_CH * HR_RS); };
+    // the operands of group gg: A row i16 = (dz, r), B column i16
+    auto operands = [&](int gg) {
+      const float* rr = rbuf((gg >> 1) & 1) + (4 * (gg & 1) + k4) * HR_RS;
+      const float* zp = rr + 24 + 8 * (i16 >> 3);
+      Ops o;
+      o.ay = rr[i16 & 7];
+      o.bx = rr[8 + i16];
+      o.z4 = *reinterpret_cast<const float4*>(zp);
+      o.z5 = zp[4];
+      return o;
+    };
+    int ce[HR_NK];
+#pragma unroll
+    for (int K = 0; K < HR_NK; ++K) ce[K] = __builtin_amdgcn_readfirstlane(wkend[K]) / HR_CH;
+    fetch(0);
+    load();
+    values(rbuf(0));
+    fetch(HR_CH);
+    load();
+    fetch(2 * HR_CH);
+    // chunk c = groups 2c (operands in oa) and 2c + 1 (in ob): each set is
+    // read a group or more before its products
+    Ops oa = operands(0), ob;
+    int c = 0;
+    // one wavefront per window: its LDS accesses complete in program order,
+    // so the chunk buffers need no barrier
+    hm_static_for<0, 8>([&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      hm_static_for<0, 4>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        for (; c < ce[R * 4 + J]; ++c) {
+          ob = operands(2 * c + 1);
+          // values of chunk c + 1, record of chunk c + 2, permutation entry
+          // of chunk c + 3
+          values(rbuf((c + 1) & 1));
+          load();
+          fetch((c + 3) * HR_CH);
+          hr_products<J>(acc, oa.ay, oa.bx, oa.z4, oa.z5);
+          oa = operands(2 * c + 2);
+          hr_products<J>(acc, ob.ay, ob.bx, ob.z4, ob.z5);
+        }
+      });
+      // the chunk buffer not holding chunk c is free here
+      hr_flush<R>(acc, out, rbuf((c + 1) & 1), R, lane, add);
+    });
+    // rows 8..14 (ring rows 0..6); ring row 7 (tile row 15) stays empty
+    hm_static_for<0, 7>([&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      hr_flush<R>(acc, out, rbuf((c + 1) & 1), 8 + R, lane, add);
+    });
+  }
+}
+
 // tabs[a][c * 4 + i] = (bin coordinate << 6) | u of the i-th halo tile
 // covering fine-grid coordinate c along dimension a, -1 terminated
 template <typename T, int TY>
@@ -2378,6 +2710,71 @@ __global__ void __launch_bounds__(256)
             const int bin = (exa[ix] >> 6) + nt0 * (eya[iy] >> 6) + nt01 * (eza[iz] >> 6);
             const int off = ((eza[iz] & 63) * HS_Y + (eya[iy] & 63)) * HS_X + (exa[ix] & 63);
             for (int sl = 0; sl < P; ++sl) acc += halo[(bin * P + sl) * HT + off];
+          }
+    }
+    grid[i] = acc;
+  }
+}
+
+// Merge of the ring spreader's window tiles: per (bin, slice) two windows
+// [15 rows][16 planes][16 columns] at x offsets 0 and 8 of the bin.  Windows
+// follow each other with stride 8 in x as the bins do in y and z, so a cell
+// is covered by at most two windows, two bin rows and two bin planes (every
+// cell unless a partial last tile sits next to the wrap); its up to 8 P
+// loads are issued together and summed in a fixed order (z, y, x entries,
+// then slices).  The x entries come from the window table (window index
+// 2 bx + h); 32-bit offsets (the host checks the halo buffer size).
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_merge_win(const float* __restrict__ halo, const int* __restrict__ tabs, SpreadDev<float> a,
+                float* __restrict__ grid) {
+  constexpr int WT = HR_WT;
+  constexpr int BT = 2 * P * WT;  // floats per bin
+  const int nfx = a.nf[0], nfy = a.nf[1], nfz = a.nf[2];
+  const int64_t n = (int64_t)nfx * nfy * nfz;
+  const int4* ty = reinterpret_cast<const int4*>(tabs) + nfx;
+  const int4* tz = ty + nfy;
+  const int4* tw = tz + nfz;
+  const int nt0 = a.ntile[0], nt01 = a.ntile[0] * a.ntile[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int gx = (int)(i % nfx);
+    const int64_t r = i / nfx;
+    const int gy = (int)(r % nfy);
+    const int gz = (int)(r / nfy);
+    const int4 ex = tw[gx], ey = ty[gy], ez = tz[gz];
+    float acc = 0.f;
+    if (ex.z < 0 && ey.z < 0 && ez.z < 0) {
+      const int ox[2] = {(ex.x >> 7) * BT + ((ex.x >> 6) & 1) * WT + (ex.x & 63),
+                         (ex.y >> 7) * BT + ((ex.y >> 6) & 1) * WT + (ex.y & 63)};
+      const int oy[2] = {(ey.x >> 6) * nt0 * BT + (ey.x & 63) * 256, (ey.y >> 6) * nt0 * BT + (ey.y & 63) * 256};
+      const int oz[2] = {(ez.x >> 6) * nt01 * BT + (ez.x & 63) * 16, (ez.y >> 6) * nt01 * BT + (ez.y & 63) * 16};
+      const bool vx1 = ex.y >= 0, vy1 = ey.y >= 0, vz1 = ez.y >= 0;
+      float v[8][P];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int iz = c >> 2, iy = (c >> 1) & 1, ix = c & 1;
+        const bool ok = (ix == 0 || vx1) && (iy == 0 || vy1) && (iz == 0 || vz1);
+        const int base = ox[ix] + oy[iy] + oz[iz];
+#pragma unroll
+        for (int s = 0; s < P; ++s) v[c][s] = ok ? halo[base + s * 2 * WT] : 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int iz = c >> 2, iy = (c >> 1) & 1, ix = c & 1;
+        if ((ix == 0 || vx1) && (iy == 0 || vy1) && (iz == 0 || vz1)) {
+#pragma unroll
+          for (int s = 0; s < P; ++s) acc += v[c][s];
+        }
+      }
+    } else {
+      const int exa[4] = {ex.x, ex.y, ex.z, ex.w}, eya[4] = {ey.x, ey.y, ey.z, ey.w};
+      const int eza[4] = {ez.x, ez.y, ez.z, ez.w};
+      for (int iz = 0; iz < 4 && eza[iz] >= 0; ++iz)
+        for (int iy = 0; iy < 4 && eya[iy] >= 0; ++iy)
+          for (int ix = 0; ix < 4 && exa[ix] >= 0; ++ix) {
+            const int bin = (exa[ix] >> 7) + nt0 * (eya[iy] >> 6) + nt01 * (eza[iz] >> 6);
+            const int off = ((exa[ix] >> 6) & 1) * WT + ((eya[iy] & 63) * 16 + (eza[iz] & 63)) * 16 + (exa[ix] & 63);
+            for (int s = 0; s < P; ++s) acc += halo[bin * BT + s * 2 * WT + off];
           }
     }
     grid[i] = acc;
@@ -2740,10 +3137,22 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
                      p.bin_start.get(), a, sp.halo_buf.get())
     bool done = false;
     if constexpr (sizeof(T) == 4) {
+      if ((sp.mode == Spreader<T>::HALOM || sp.mode == Spreader<T>::HALOR) && t8) {
+        const size_t need = (size_t)g.nbins * (sp.halo_parts / 2) *
+                            (sp.mode == Spreader<T>::HALOM ? (size_t)HaloShape<8>::ELEMS : 2 * (size_t)HR_WT);
+        if (sp.halo_parts < 2 || need > sp.halo_buf.size())
+          throw RuntimeFailure("matrix-core halo spreader: halo buffer too small for the slices");
+      }
       if (sp.mode == Spreader<T>::HALOM && t8) {
         // one tile per (bin, slice): halo_parts counts 2 windows per slice
         const dim3 grd2(g.nbins, sp.halo_parts / 2);
         hipLaunchKernelGGL(k_spread3_halo_mfma2<1024>, grd2, dim3(128), 0, s, p.scell.get(), p.rec.get(),
+                           p.bin_start.get(), a, sp.halo_buf.get());
+        done = true;
+      } else if (sp.mode == Spreader<T>::HALOR && t8) {
+        // one tile per (bin, slice, window)
+        const dim3 grd2(g.nbins, sp.halo_parts / 2);
+        hipLaunchKernelGGL(k_spread3_halo_ring<1024>, grd2, dim3(128), 0, s, p.scell.get(), p.rec.get(),
                            p.bin_start.get(), a, sp.halo_buf.get());
         done = true;
       }
@@ -2767,6 +3176,22 @@ static bool launch_spread3_halo(const Spreader<T>& sp, const SortedPoints<T>& p,
   ProfScope pm(ST_MERGE, s);
   const size_t n = (size_t)g.ncells();
   const dim3 grd((unsigned)std::min<size_t>(blocks_for(n, 256), 16384));
+  if constexpr (sizeof(T) == 4) {
+    if (sp.mode == Spreader<T>::HALOR && t8) {
+      const int sls = sp.halo_parts / 2;
+      if ((size_t)g.nbins * sp.halo_parts * HR_WT >= (1u << 31))
+        throw RuntimeFailure("halo merge: window tiles exceed 32-bit offsets");
+      switch (sls) {
+        case 1: hipLaunchKernelGGL(k_merge_win<1>, grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, grid); break;
+        case 2: hipLaunchKernelGGL(k_merge_win<2>, grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, grid); break;
+        case 3: hipLaunchKernelGGL(k_merge_win<3>, grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, grid); break;
+        case 4: hipLaunchKernelGGL(k_merge_win<4>, grd, dim3(256), 0, s, sp.halo_buf.get(), sp.halo_tab.get(), a, grid); break;
+        default: throw RuntimeFailure("halo merge: ring spreader slices must be 1..4");
+      }
+      FRESCO_HIP_CHECK(hipGetLastError());
+      return true;
+    }
+  }
   const size_t hel = t8 ? HaloShape<8>::ELEMS : HaloShape<16>::ELEMS;
   // halo tiles per bin (the matrix-core spreader counts 2 windows per tile)
   const int tiles = sp.mode == Spreader<T>::HALOM && t8 ? sp.halo_parts / 2 : sp.halo_parts;

@@ -145,12 +145,14 @@ class Spreader {
   // input-driven halo spreader (3D, w <= 8)
   // HALO4: the halo spreader with four wavefronts per tile (z-planes split
   // by residue mod 4); HALOM: FP32 halo spreading as matrix products of the
-  // points that share a z support, two x windows per tile
-  enum Mode { REG = 0, HALO = 1, HALO4 = 2, HALOM = 3 };
+  // points that share a z support, two x windows per tile; HALOR: matrix
+  // products of the points that share a y support, the y rows kept in a
+  // ring, one tile per x window
+  enum Mode { REG = 0, HALO = 1, HALO4 = 2, HALOM = 3, HALOR = 4 };
   mutable int mode = REG;
   bool halo_ok = false;
   mutable bool tuned = true;   // false: time the candidates on the first spread
-  mutable int halo_parts = 1;  // point-range slices per bin (HALOM: 2 windows per slice)
+  mutable int halo_parts = 1;  // point-range slices per bin (HALOM, HALOR: 2 windows per slice)
   int max_halo_parts = 1;
   DeviceBuffer<T> halo_buf;    // [nbins][halo_parts][halo tile]
   DeviceBuffer<int> halo_tab;  // per-dimension coverage tables for the merge

@@ -236,7 +236,8 @@ class BasePotential:
         src = ctypes.c_int()
         _lib.check(lib.fresco_pot_spread_source(self._h, ctypes.byref(src)))
         return {"nf": list(nf), "w": int(w.value), "beta": float(beta.value), "tile": list(tile),
-                "spreader": {0: "tiles", 1: "halo", 2: "halo4", 3: "halom"}.get(mode.value), "spread_parts": int(parts.value),
+                "spreader": {0: "tiles", 1: "halo", 2: "halo4", 3: "halom", 4: "halor"}.get(mode.value),
+                "spread_parts": int(parts.value),
                 "spread_source": {0: "fixed", 1: "tuning table", 2: "autotuned", 3: "pinned"}.get(src.value)}
 
     def __del__(self):

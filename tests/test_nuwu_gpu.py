@@ -270,13 +270,15 @@ def _clustered(N, frac, seed):
 
 @pytest.mark.parametrize("case", ["uniform", "clustered"])
 def test_matrix_core_spreader(NUwU, case, monkeypatch):
-    """The FP32 halo spreading as matrix products (points ranked by the z of
-    their first cell in each x window, the window's planes accumulated in
-    registers), at one and two slices per bin: parity with the exact sums,
+    """The FP32 halo spreading as matrix products, at one and two slices per
+    bin: points ranked by the z of their first cell in each x window, the
+    window's planes accumulated in registers (halom), and points ranked by
+    the y and z of their first cell, the y rows kept in a ring and written
+    row by row, one tile per window (halor).  Parity with the exact sums,
     agreement with the LDS halo spreader to FP32 rounding, bitwise repeats.
     The clustered case puts over a thousand points into single bins (several
-    ranking windows per slice, runs padded to whole chunks) and leaves most
-    bins empty."""
+    ranking rounds per slice, the later rounds adding to the tiles, runs
+    padded to whole groups) and leaves most bins empty."""
     n, N = 21, 6000
     radii, Sk, V, p = make_case(3, n, N, seed=81)
     if case == "clustered":
@@ -284,7 +286,7 @@ def test_matrix_core_spreader(NUwU, case, monkeypatch):
     ref = O.NUwU(radii, 0.0, Sk, V, np.ones(3))
     E0, g0 = ref.get_energy_gradient(p)
     res = {}
-    for mode in ("halom:2", "halom:4", "halo4:2"):
+    for mode in ("halom:2", "halom:4", "halor:2", "halor:4", "halo4:2"):
         monkeypatch.setenv("FRESCO_SPREAD3", mode)
         pot = NUwU(radii, 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
         info = pot.plan_info()
@@ -297,9 +299,35 @@ def test_matrix_core_spreader(NUwU, case, monkeypatch):
         E2, g2 = pot.get_energy_gradient(p)
         assert E == E2 and np.array_equal(g, g2)
         res[mode] = (E, g)
-    for mode in ("halom:2", "halom:4"):
+    for mode in ("halom:2", "halom:4", "halor:2", "halor:4"):
         assert abs(res[mode][0] - res["halo4:2"][0]) / abs(E0) < 2e-6
         assert rel(res[mode][1], res["halo4:2"][1]) < 2e-6
+
+
+def test_ring_spreader_with_many_bins(NUwU, monkeypatch):
+    """A plan with more than 16384 bins (every mode constrained: the full
+    129^3 mode grid), where the halo buffer holds a single halo tile per bin:
+    the ring spreader's two window tiles per bin fit it, and the gradient
+    agrees with the four-wavefront LDS spreader's to FP32 rounding."""
+    n, N = 129, 20000
+    rng = np.random.default_rng(83)
+    p = rng.uniform(0, 1, 3 * N)
+    M = n ** 3
+    Sk = np.zeros(M)
+    V = np.ones(M)
+    res = {}
+    for mode in ("halor:2", "halo4:1"):
+        monkeypatch.setenv("FRESCO_SPREAD3", mode)
+        pot = NUwU(np.ones(N), 0.0, Sk, V, np.ones(3), 1e-6, precision="fp32")
+        info = pot.plan_info()
+        print(mode, info, flush=True)
+        nb = [-(-info["nf"][a] // info["tile"][a]) for a in range(3)]
+        assert nb[0] * nb[1] * nb[2] > 16384
+        assert (info["spreader"], info["spread_parts"]) == (mode.split(":")[0], int(mode[-1]))
+        res[mode] = pot.get_energy_gradient(p)
+    E0, g0 = res["halo4:1"]
+    assert abs(res["halor:2"][0] - E0) / abs(E0) < 2e-6
+    assert rel(res["halor:2"][1], g0) < 2e-6
 
 
 @pytest.mark.parametrize("case", ["uniform", "clustered"])
