@@ -2110,7 +2110,7 @@ __global__ void __launch_bounds__(128)
   __shared__ int run[2][HM_NK];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int h = tid >> 6;
+  const int h = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int b = blockIdx.x;
   const int nsl = gridDim.y, sl = blockIdx.y;
   const int beg0 = bin_start[b];
@@ -2139,20 +2139,26 @@ __global__ void __launch_bounds__(128)
   for (int z = 0; z < HP_Z; ++z) acc[z] = hm_f4{0.f, 0.f, 0.f, 0.f};
   for (int ws = beg; ws < end; ws += WIN) {
     const int wn = min(WIN, end - ws);
+    // the thread index, opaque here: the staging and ranking addresses are
+    // formed in every round instead of being kept in registers across the
+    // products (a slice rarely takes more than one round)
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const int lq = tq & 63;
     {
       float4 wr[SPL];
       uint32_t wk[SPL];
 #pragma unroll
       for (int u = 0; u < SPL; ++u) {
-        const int e = u * 128 + tid;
+        const int e = u * 128 + tq;
         wk[u] = e < wn ? scell[ws + e] : 0u;
         wr[u] = e < wn ? rec[ws + e] : float4{0.f, 0.f, 0.f, 0.f};
       }
       __syncthreads();  // both windows are done with the previous round's records
 #pragma unroll
       for (int u = 0; u < SPL; ++u) {
-        wrec[u * 128 + tid] = wr[u];
-        wsc[u * 128 + tid] = wk[u];
+        wrec[u * 128 + tq] = wr[u];
+        wsc[u * 128 + tq] = wk[u];
       }
     }
     if (lane < HM_NK) wrun[lane] = 0;
@@ -2161,7 +2167,7 @@ __global__ void __launch_bounds__(128)
     int wkey[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
-      const int e = u * 64 + lane;
+      const int e = u * 64 + lq;
       const uint32_t sc = wsc[e];
       const bool mine = e < wn && ((rel(sc, 0) >> 3) & 1) == h;
       wkey[u] = mine ? (rel(sc, 2) & 7) : -1;
@@ -2175,17 +2181,17 @@ __global__ void __launch_bounds__(128)
     int inc = pad;
 #pragma unroll
     for (int off = 1; off < HM_NK; off <<= 1) {
-      const int u = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += u;
+      const int u = __builtin_amdgcn_ds_bpermute(max(lq - off, 0) << 2, inc);
+      if (lq >= off) inc += u;
     }
-    const int nslot = __builtin_amdgcn_readfirstlane(__shfl(inc, HM_NK - 1, 64));
+    const int nslot = __builtin_amdgcn_readlane(inc, HM_NK - 1);
     const int st = inc - pad;
     wave_lds_order();
     if (lane < HM_NK) {
       wrun[lane] = st;
       wkend[lane] = st + pad;
     }
-    for (int q = lane; q < nslot; q += 64) wperm[q] = 0xFFFF;
+    for (int q = lq; q < nslot; q += 64) wperm[q] = 0xFFFF;
     wave_lds_order();
     // stable placement, 64 points a round per wavefront
 #pragma unroll
@@ -2201,7 +2207,7 @@ __global__ void __launch_bounds__(128)
         same &= bit ? mk : ~mk;
       }
       const int rank = __popcll(same & lt);
-      if (valid) wperm[min(wrun[key] + rank, MAXSLOT - 1)] = (uint16_t)(u * 64 + lane);
+      if (valid) wperm[min(wrun[key] + rank, MAXSLOT - 1)] = (uint16_t)(u * 64 + lq);
       wave_lds_order();
       if (valid && rank == 0) wrun[key] += __popcll(same);
       wave_lds_order();
