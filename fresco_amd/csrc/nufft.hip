@@ -3579,11 +3579,19 @@ __global__ void __launch_bounds__(G8_THREADS)
 #pragma unroll
       for (int jc = 0; jc < 8; ++jc) {
         T pA[2] = {T(0), T(0)}, pB[2] = {T(0), T(0)};
+        // the plane's 16 cells are read in one batch (the compiler barrier
+        // keeps the reads together: 8 paired reads in flight instead of 2)
+        T gv[8][2];
+#pragma unroll
+        for (int jb = 0; jb < 8; ++jb)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) gv[jb][h] = col[jc * zs + jb * srx + 4 * h];
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int jb = 0; jb < 8; ++jb) {
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const T g = col[jc * zs + jb * srx + 4 * h];
+            const T g = gv[jb][h];
             pA[h] += g * wy[jb];
             if (DERIV) pB[h] += g * dwy[jb];
           }

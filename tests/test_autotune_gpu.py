@@ -81,4 +81,4 @@ def test_shipped_table_fixes_the_n2p20_configuration(monkeypatch):
     info = pot.plan_info()
     print("3D N=2^20 spreader:", info, flush=True)
     assert info["spread_source"] == "tuning table"
-    assert (info["spreader"], info["spread_parts"]) == ("halom", 2)
+    assert (info["spreader"], info["spread_parts"]) == ("halom", 4)
