@@ -2673,7 +2673,12 @@ __global__ void __launch_bounds__(256)
   const int4* ty = tx + nfx;
   const int4* tz = ty + nfy;
   const int nt0 = a.ntile[0], nt01 = a.ntile[0] * a.ntile[1];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // workgroups go round-robin to the 8 XCDs; each XCD takes a contiguous
+  // range of cells (a slab of z planes), so the tiles it reads are shared in
+  // its own L2
+  const int nbk = gridDim.x, xq = nbk >> 3, xr = nbk & 7, xc = blockIdx.x & 7;
+  const int blk = xc * xq + (xc < xr ? xc : xr) + (blockIdx.x >> 3);
+  for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int gx = (int)(i % nfx);
     const int64_t r = i / nfx;
     const int gy = (int)(r % nfy);
