@@ -493,6 +493,175 @@ __global__ void __launch_bounds__(PB_T)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Segmented direct binning: the points are stored pattern by pattern and each
+// pattern's own bins fit PB_MAXB.  Tiles of up to PB_TILE points never
+// straddle a pattern (tile t: points [tbeg[t], tbeg[t + 1]) of pattern
+// tpat[t]), so counts, prefixes and places are per pattern over its local
+// bins; the result is the same stable order as the segmented radix sort.
+//   k_bin_tiles_seg  per tile: records in the original order and the
+//                    tile's local bin counts (cnt[tile * nbins + bin])
+//   k_bin_scan_seg   per pattern: exclusive prefix of each local bin's
+//                    counts over the pattern's tiles (in place), then of the
+//                    bin totals over the bins: bin_start
+//   k_bin_place_seg  per tile: k_bin_place over local bins
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(PB_CT)
+    k_bin_tiles_seg(const double* __restrict__ x, MapDev m, const uint32_t* __restrict__ pat, uint32_t nbins,
+                    const T* __restrict__ c, const int* __restrict__ tbeg, const int* __restrict__ tpat,
+                    uint32_t* __restrict__ key, uint32_t* __restrict__ cell_out,
+                    typename Vec4<T>::type* __restrict__ rec_out, uint32_t* __restrict__ cnt) {
+  constexpr int PER = PB_TILE / PB_CT;
+  __shared__ uint32_t h[PB_MAXB];
+  const int t = threadIdx.x;
+  const int nb = (int)nbins;
+  for (int b = t; b < nb; b += PB_CT) h[b] = 0;
+  const int j0 = tbeg[blockIdx.x], j1 = tbeg[blockIdx.x + 1];
+  const uint32_t base = (uint32_t)tpat[blockIdx.x] * nbins;
+  uint32_t k[PER], packed[PER];
+  typename Vec4<T>::type rc[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int j = j0 + r * PB_CT + t;
+    if (j < j1) k[r] = point_record(x, j, m, pat[j] * nbins, c, packed[r], rc[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int j = j0 + r * PB_CT + t;
+    if (j < j1) {
+      key[j] = k[r];
+      cell_out[j] = packed[r];
+      rec_out[j] = rc[r];
+      atomicAdd(&h[k[r] - base], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = t; b < nb; b += PB_CT) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
+}
+
+constexpr int PB_SGT = 1024;  // threads of k_bin_scan_seg: two local bins each at most
+static_assert(PB_MAXB <= 2 * PB_SGT, "a pattern's bins are scanned two per thread");
+
+// grid: patterns; ptile[p] .. ptile[p + 1] the pattern's tiles, pstart[p] its
+// first point; start[npat * nbins] = n is written by the last pattern
+__global__ void __launch_bounds__(PB_SGT)
+    k_bin_scan_seg(uint32_t* __restrict__ cnt, const int* __restrict__ ptile, const int* __restrict__ pstart,
+                   int nbins, int npat, int64_t n, int* __restrict__ start) {
+  __shared__ uint32_t wsum[PB_SGT / 64];
+  const int p = blockIdx.x, t = threadIdx.x;
+  const int t0 = ptile[p], t1 = ptile[p + 1];
+  // the running prefix of local bins b = 2t and 2t + 1 over the tiles, 16
+  // loads in flight per bin
+  constexpr int U = 16;
+  uint32_t tot[2] = {0u, 0u};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int b = 2 * t + q;
+    if (b >= nbins) continue;
+    uint32_t off = 0;
+    for (int tt = t0; tt < t1; tt += U) {
+      uint32_t v[U];
+#pragma unroll
+      for (int i = 0; i < U; ++i) v[i] = tt + i < t1 ? cnt[(int64_t)(tt + i) * nbins + b] : 0u;
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        if (tt + i < t1) cnt[(int64_t)(tt + i) * nbins + b] = off;
+        off += v[i];
+      }
+    }
+    tot[q] = off;
+  }
+  // exclusive scan of the bin totals (wavefront scan, then the wavefront
+  // totals)
+  const int lane = t & 63;
+  uint32_t inc = tot[0] + tot[1];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[t >> 6] = inc;
+  __syncthreads();
+  uint32_t ex = inc - (tot[0] + tot[1]);
+  for (int w = 0; w < (t >> 6); ++w) ex += wsum[w];
+  const int base = pstart[p];
+  if (2 * t < nbins) start[(int64_t)p * nbins + 2 * t] = base + (int)ex;
+  if (2 * t + 1 < nbins) start[(int64_t)p * nbins + 2 * t + 1] = base + (int)(ex + tot[0]);
+  if (p == npat - 1 && t == 0) start[(int64_t)npat * nbins] = (int)n;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PB_T)
+    k_bin_place_seg(const uint32_t* __restrict__ key, const uint32_t* __restrict__ cell_in,
+                    const typename Vec4<T>::type* __restrict__ rec_in, const int* __restrict__ tbeg,
+                    const int* __restrict__ tpat, int nbins, int dbits, const uint32_t* __restrict__ cnt,
+                    const int* __restrict__ start, uint32_t* __restrict__ idx, uint32_t* __restrict__ cell_out,
+                    typename Vec4<T>::type* __restrict__ rec_out) {
+  __shared__ uint32_t run[PB_MAXB];
+  __shared__ uint32_t wc[PB_T / 64][PB_MAXB];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  // XCD-contiguous tile ranges (as k_bin_place)
+  const int nt = gridDim.x, xq = nt >> 3, xr = nt & 7, xc = blockIdx.x & 7;
+  const int tile = xc * xq + (xc < xr ? xc : xr) + (blockIdx.x >> 3);
+  const int64_t trow = (int64_t)tile * nbins;
+  const uint32_t kbase = (uint32_t)tpat[tile] * (uint32_t)nbins;
+  const int64_t sbase = (int64_t)tpat[tile] * nbins;
+  for (int b = t; b < nbins; b += PB_T) {
+    run[b] = (uint32_t)start[sbase + b] + cnt[trow + b];
+#pragma unroll
+    for (int w = 0; w < PB_T / 64; ++w) wc[w][b] = 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int j0 = tbeg[tile], j1 = tbeg[tile + 1];
+  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS];
+  typename Vec4<T>::type rcs[PB_ROUNDS];
+#pragma unroll
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int j = j0 + r * PB_T + t;
+    ks[r] = 0;
+    cls[r] = 0;
+    if (j < j1) {
+      ks[r] = key[j] - kbase;
+      cls[r] = cell_in[j];
+      rcs[r] = rec_in[j];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int j = j0 + r * PB_T + t;
+    const bool valid = j < j1;
+    const uint32_t k = ks[r], cl = cls[r];
+    const typename Vec4<T>::type rc = rcs[r];
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+      const bool bit = (k >> b) & 1u;
+      const uint64_t mk = __ballot(bit);
+      same &= bit ? mk : ~mk;
+    }
+    const int rank = __popcll(same & lt);
+    if (valid && rank == 0) wc[wv][k] = (uint32_t)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[k] + (uint32_t)rank;
+      for (int w = 0; w < wv; ++w) pos += wc[w][k];
+      idx[pos] = (uint32_t)j;
+      cell_out[pos] = cl;
+      rec_out[pos] = rc;
+    }
+    __syncthreads();
+    if (valid && rank == 0) {
+      atomicAdd(&run[k], wc[wv][k]);
+      wc[wv][k] = 0;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename K>
 __global__ void k_bin_start(const K* __restrict__ keys, int64_t n, int nbins, int* __restrict__ start) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -575,6 +744,40 @@ void SortedPoints<T>::init_sorter() {
 
 template <typename T>
 void SortedPoints<T>::set_segments(const std::vector<int64_t>& starts) {
+  if (starts.size() < 2 || starts.front() != 0 || starts.back() != n)
+    throw InvalidArgument("point segments must cover the points");
+  if ((int64_t)starts.size() - 1 != geom.batch) throw InvalidArgument("one point segment per pattern");
+  // FRESCO_DIRECT_BINNING=0 keeps the segmented radix sort (the same order)
+  const char* env = std::getenv("FRESCO_DIRECT_BINNING");
+  if (geom.nbins <= PB_MAXB && !(env && env[0] == '0')) {
+    std::vector<int> tb, tp, pt, ps;
+    for (size_t p = 0; p + 1 < starts.size(); ++p) {
+      if (starts[p + 1] < starts[p]) throw InvalidArgument("point segments must be ordered");
+      pt.push_back((int)tp.size());
+      ps.push_back((int)starts[p]);
+      for (int64_t b = starts[p]; b < starts[p + 1]; b += PB_TILE) {
+        tb.push_back((int)b);
+        tp.push_back((int)p);
+      }
+    }
+    pt.push_back((int)tp.size());
+    ps.push_back((int)n);
+    tb.push_back((int)n);
+    direct = false;
+    seg_direct_ = true;
+    seg_ntiles_ = (int)tp.size();
+    seg_tbeg_.allocate(tb.size());
+    seg_tbeg_.upload(tb.data(), tb.size());
+    seg_tpat_.allocate(std::max<size_t>(1, tp.size()));
+    if (!tp.empty()) seg_tpat_.upload(tp.data(), tp.size());
+    seg_ptile_.allocate(pt.size());
+    seg_ptile_.upload(pt.data(), pt.size());
+    seg_pstart_.allocate(ps.size());
+    seg_pstart_.upload(ps.data(), ps.size());
+    tcnt_.allocate(std::max<size_t>(1, (size_t)seg_ntiles_ * geom.nbins));
+    return;
+  }
+  seg_direct_ = false;
   if (direct) init_sorter();
   sorter.set_segments(starts, geom.nbins);
 }
@@ -595,6 +798,19 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
     hipLaunchKernelGGL(k_bin_bases, dim3(1), dim3(PB_BT), 0, s, gsum_.get(), ngroups_, tb, n, bin_start.get());
     hipLaunchKernelGGL(k_bin_place<T>, dim3(ntiles_), dim3(PB_T), 0, s, key.get(), scell_in.get(), rec_in.get(), n,
                        tb, dbits, tcnt_.get(), gsum_.get(), gsz_, bin_start.get(), idx.get(), scell.get(), rec.get());
+    FRESCO_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (seg_direct_) {
+    int dbits = 1;
+    while ((1 << dbits) < geom.nbins) ++dbits;
+    hipLaunchKernelGGL(k_bin_tiles_seg<T>, dim3(seg_ntiles_), dim3(PB_CT), 0, s, x, m, pat, (uint32_t)geom.nbins, c,
+                       seg_tbeg_.get(), seg_tpat_.get(), key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
+    hipLaunchKernelGGL(k_bin_scan_seg, dim3(geom.batch), dim3(PB_SGT), 0, s, tcnt_.get(), seg_ptile_.get(),
+                       seg_pstart_.get(), geom.nbins, geom.batch, n, bin_start.get());
+    hipLaunchKernelGGL(k_bin_place_seg<T>, dim3(seg_ntiles_), dim3(PB_T), 0, s, key.get(), scell_in.get(),
+                       rec_in.get(), seg_tbeg_.get(), seg_tpat_.get(), geom.nbins, dbits, tcnt_.get(), bin_start.get(),
+                       idx.get(), scell.get(), rec.get());
     FRESCO_HIP_CHECK(hipGetLastError());
     return;
   }

@@ -106,6 +106,11 @@ class SortedPoints {
   void init_sorter();
   int ntiles_ = 0, gsz_ = 16, ngroups_ = 0;
   DeviceBuffer<uint32_t> tcnt_, gsum_;  // direct binning: per-tile counts, per-group totals
+  // segmented direct binning (points stored pattern by pattern, each
+  // pattern's bins fit one counting pass): tiles within patterns
+  bool seg_direct_ = false;
+  int seg_ntiles_ = 0;
+  DeviceBuffer<int> seg_tbeg_, seg_tpat_, seg_ptile_, seg_pstart_;
 };
 
 template <typename T>
