@@ -592,6 +592,17 @@ __global__ void __launch_bounds__(PB_SGT)
   if (p == npat - 1 && t == 0) start[(int64_t)npat * nbins] = (int)n;
 }
 
+// k_bin_place over the tile's local bins, with the tile's records staged in
+// LDS in bin order before they go out: consecutive threads then write each
+// bin's run (a few records per bin and tile) to consecutive places, instead
+// of every lane writing to a different partial line.  Dynamic LDS layout:
+// records [PB_TILE], cells, indices and destinations [PB_TILE] each, then
+// run, run0, lst [nbins] each and wc [PB_T / 64][nbins].
+template <typename R4>
+__host__ __device__ constexpr size_t place_seg_lds_bytes(int nbins) {
+  return (size_t)PB_TILE * (sizeof(R4) + 3 * 4) + (size_t)(3 + PB_T / 64) * nbins * 4;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(PB_T)
     k_bin_place_seg(const uint32_t* __restrict__ key, const uint32_t* __restrict__ cell_in,
@@ -599,8 +610,17 @@ __global__ void __launch_bounds__(PB_T)
                     const int* __restrict__ tpat, int nbins, int dbits, const uint32_t* __restrict__ cnt,
                     const int* __restrict__ start, uint32_t* __restrict__ idx, uint32_t* __restrict__ cell_out,
                     typename Vec4<T>::type* __restrict__ rec_out) {
-  __shared__ uint32_t run[PB_MAXB];
-  __shared__ uint32_t wc[PB_T / 64][PB_MAXB];
+  using R4 = typename Vec4<T>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  R4* srec = reinterpret_cast<R4*>(smem);
+  uint32_t* scl = reinterpret_cast<uint32_t*>(srec + PB_TILE);
+  uint32_t* sidx = scl + PB_TILE;
+  uint32_t* sdst = sidx + PB_TILE;
+  uint32_t* run = sdst + PB_TILE;  // [nbins] next place of each bin
+  uint32_t* run0 = run + nbins;    // [nbins] the tile's first place of each bin
+  uint32_t* lst = run0 + nbins;    // [nbins] the bin's first slot in the staged tile
+  uint32_t* wc = lst + nbins;      // [PB_T / 64][nbins] per-wavefront counts of a round
+  __shared__ uint32_t wsum[PB_T / 64];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
@@ -611,15 +631,17 @@ __global__ void __launch_bounds__(PB_T)
   const uint32_t kbase = (uint32_t)tpat[tile] * (uint32_t)nbins;
   const int64_t sbase = (int64_t)tpat[tile] * nbins;
   for (int b = t; b < nbins; b += PB_T) {
-    run[b] = (uint32_t)start[sbase + b] + cnt[trow + b];
+    const uint32_t r0 = (uint32_t)start[sbase + b] + cnt[trow + b];
+    run[b] = r0;
+    run0[b] = r0;
 #pragma unroll
-    for (int w = 0; w < PB_T / 64; ++w) wc[w][b] = 0;
+    for (int w = 0; w < PB_T / 64; ++w) wc[w * nbins + b] = 0;
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int j0 = tbeg[tile], j1 = tbeg[tile + 1];
-  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS];
-  typename Vec4<T>::type rcs[PB_ROUNDS];
+  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS], dst[PB_ROUNDS];
+  R4 rcs[PB_ROUNDS];
 #pragma unroll
   for (int r = 0; r < PB_ROUNDS; ++r) {
     const int j = j0 + r * PB_T + t;
@@ -631,12 +653,13 @@ __global__ void __launch_bounds__(PB_T)
       rcs[r] = rec_in[j];
     }
   }
+  // each point's place: rounds of 256 points in index order, ranks among
+  // equal bins by ballots
 #pragma unroll
   for (int r = 0; r < PB_ROUNDS; ++r) {
     const int j = j0 + r * PB_T + t;
     const bool valid = j < j1;
-    const uint32_t k = ks[r], cl = cls[r];
-    const typename Vec4<T>::type rc = rcs[r];
+    const uint32_t k = ks[r];
     uint64_t same = __ballot(valid);
     for (int b = 0; b < dbits; ++b) {
       const bool bit = (k >> b) & 1u;
@@ -644,21 +667,61 @@ __global__ void __launch_bounds__(PB_T)
       same &= bit ? mk : ~mk;
     }
     const int rank = __popcll(same & lt);
-    if (valid && rank == 0) wc[wv][k] = (uint32_t)__popcll(same);
+    if (valid && rank == 0) wc[wv * nbins + k] = (uint32_t)__popcll(same);
     __syncthreads();
+    dst[r] = 0;
     if (valid) {
       uint32_t pos = run[k] + (uint32_t)rank;
-      for (int w = 0; w < wv; ++w) pos += wc[w][k];
-      idx[pos] = (uint32_t)j;
-      cell_out[pos] = cl;
-      rec_out[pos] = rc;
+      for (int w = 0; w < wv; ++w) pos += wc[w * nbins + k];
+      dst[r] = pos;
     }
     __syncthreads();
     if (valid && rank == 0) {
-      atomicAdd(&run[k], wc[wv][k]);
-      wc[wv][k] = 0;
+      atomicAdd(&run[k], wc[wv * nbins + k]);
+      wc[wv * nbins + k] = 0;
     }
     __syncthreads();
+  }
+  // the tile's bin counts (run - run0), their exclusive prefix over the
+  // bins: thread t takes bins [t * per, (t + 1) * per)
+  const int per = (nbins + PB_T - 1) / PB_T;
+  uint32_t loc = 0;
+  for (int b = t * per; b < min(nbins, (t + 1) * per); ++b) loc += run[b] - run0[b];
+  uint32_t inc = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t ex = inc - loc;
+  for (int w = 0; w < wv; ++w) ex += wsum[w];
+  for (int b = t * per; b < min(nbins, (t + 1) * per); ++b) {
+    lst[b] = ex;
+    ex += run[b] - run0[b];
+  }
+  __syncthreads();
+  // stage in bin order, then write the runs with consecutive threads
+#pragma unroll
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int j = j0 + r * PB_T + t;
+    if (j < j1) {
+      const uint32_t k = ks[r];
+      const uint32_t s2 = lst[k] + (dst[r] - run0[k]);
+      srec[s2] = rcs[r];
+      scl[s2] = cls[r];
+      sidx[s2] = (uint32_t)j;
+      sdst[s2] = dst[r];
+    }
+  }
+  __syncthreads();
+  const int np = j1 - j0;
+  for (int q = t; q < np; q += PB_T) {
+    const uint32_t pos = sdst[q];
+    idx[pos] = sidx[q];
+    cell_out[pos] = scl[q];
+    rec_out[pos] = srec[q];
   }
 }
 
@@ -808,7 +871,8 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
                        seg_tbeg_.get(), seg_tpat_.get(), key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
     hipLaunchKernelGGL(k_bin_scan_seg, dim3(geom.batch), dim3(PB_SGT), 0, s, tcnt_.get(), seg_ptile_.get(),
                        seg_pstart_.get(), geom.nbins, geom.batch, n, bin_start.get());
-    hipLaunchKernelGGL(k_bin_place_seg<T>, dim3(seg_ntiles_), dim3(PB_T), 0, s, key.get(), scell_in.get(),
+    const size_t lds = place_seg_lds_bytes<typename Vec4<T>::type>(geom.nbins);
+    hipLaunchKernelGGL(k_bin_place_seg<T>, dim3(seg_ntiles_), dim3(PB_T), lds, s, key.get(), scell_in.get(),
                        rec_in.get(), seg_tbeg_.get(), seg_tpat_.get(), geom.nbins, dbits, tcnt_.get(), bin_start.get(),
                        idx.get(), scell.get(), rec.get());
     FRESCO_HIP_CHECK(hipGetLastError());
