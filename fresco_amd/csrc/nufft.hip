@@ -3336,12 +3336,39 @@ __global__ void k_merge_halo2(const T* __restrict__ halo, const int* __restrict_
   const int nbins = a.ntile[0] * a.ntile[1];
   const int* tx = tabs;
   const int* ty = tabs + 4 * a.nf[0];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // workgroups go round-robin to the 8 XCDs; each XCD takes a contiguous
+  // range of cells, so the tiles it reads are shared in its own L2
+  const int nbk = gridDim.x, xq = nbk >> 3, xr = nbk & 7, xc = blockIdx.x & 7;
+  const int blk = xc * xq + (xc < xr ? xc : xr) + (blockIdx.x >> 3);
+  for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t pat = i / nc;
     const int64_t c = i - pat * nc;
     const int gx = (int)(c % a.nf[0]);
     const int gy = (int)(c / a.nf[0]);
     T acc = T(0);
+    const int4 ex4 = reinterpret_cast<const int4*>(tx)[gx], ey4 = reinterpret_cast<const int4*>(ty)[gy];
+    if (parts == 1 && ex4.z < 0 && ey4.z < 0) {
+      // at most two tiles per dimension and one slice: the up to 4 loads
+      // issued together, summed in the loop's order (y entries, then x)
+      const int exa[2] = {ex4.x, ex4.y}, eya[2] = {ey4.x, ey4.y};
+      T v[2][2];
+#pragma unroll
+      for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+        for (int ix = 0; ix < 2; ++ix) {
+          const bool ok = eya[iy] >= 0 && exa[ix] >= 0;
+          const int64_t bin = pat * nbins + (exa[ix] >> 8) + (int64_t)a.ntile[0] * (eya[iy] >> 8);
+          const int64_t off = (int64_t)(eya[iy] & 255) * hs + (exa[ix] & 255);
+          v[iy][ix] = ok ? halo[bin * helems + off] : T(0);
+        }
+#pragma unroll
+      for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+        for (int ix = 0; ix < 2; ++ix)
+          if (eya[iy] >= 0 && exa[ix] >= 0) acc += v[iy][ix];
+      grid[i] = acc;
+      continue;
+    }
     for (int iy = 0; iy < 4; ++iy) {
       const int ey = ty[gy * 4 + iy];
       if (ey < 0) break;
