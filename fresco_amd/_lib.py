@@ -132,6 +132,7 @@ SIGNATURES = {
     "fresco_opt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fresco_prof_enable": (ctypes.c_int, [ctypes.c_int]),
     "fresco_prof_enable_stages": (ctypes.c_int, [ctypes.c_uint]),
+    "fresco_prof_sample_period": (ctypes.c_int, [ctypes.c_int]),
     "fresco_prof_reset": (ctypes.c_int, []),
     "fresco_prof_get": (
         ctypes.c_int,
@@ -164,6 +165,11 @@ def prof_enable_stages(names):
         if name.value.decode() in names:
             mask |= 1 << st
     check(lib.fresco_prof_enable_stages(mask))
+
+
+def prof_sample_period(period):
+    """Time one launch in `period` of each enabled stage (1: every launch)."""
+    check(load().fresco_prof_sample_period(int(period)))
 
 
 def prof_reset():

@@ -426,6 +426,13 @@ int fresco_prof_enable_stages(unsigned mask) {
   return guarded([&]() { fresco_hip::prof_set_mask(mask); });
 }
 
+int fresco_prof_sample_period(int period) {
+  return guarded([&]() {
+    if (period < 1) throw fresco_hip::InvalidArgument("profiler sample period must be at least 1");
+    fresco_hip::prof_set_period(period);
+  });
+}
+
 int fresco_prof_reset(void) {
   return guarded([&]() { fresco_hip::prof_reset(); });
 }

@@ -19,6 +19,8 @@ unsigned g_mask = 0;
 std::vector<hipEvent_t> g_pool;
 std::vector<Pending> g_pending;
 hipEvent_t g_open[ST_COUNT] = {};
+int g_period = 1;
+long long g_seen[ST_COUNT] = {};
 double g_ms[ST_COUNT] = {};
 long long g_cnt[ST_COUNT] = {};
 
@@ -53,11 +55,22 @@ void prof_set_mask(unsigned m) {
   g_on = g_mask != 0;
 }
 
+void prof_set_period(int k) {
+  std::lock_guard<std::mutex> l(g_mu);
+  g_period = k < 1 ? 1 : k;
+}
+
+bool prof_take(int st) {
+  std::lock_guard<std::mutex> l(g_mu);
+  return g_seen[st]++ % g_period == 0;
+}
+
 void prof_reset() {
   std::lock_guard<std::mutex> l(g_mu);
   for (int i = 0; i < ST_COUNT; ++i) {
     g_ms[i] = 0;
     g_cnt[i] = 0;
+    g_seen[i] = 0;
   }
 }
 

@@ -15,6 +15,9 @@ bool prof_stage_on(int st);      // this stage timed
 void prof_set(bool on);          // all stages / none
 void prof_set_mask(unsigned m);  // the stages whose bit is set
 unsigned prof_mask();
+void prof_set_period(int k);     // time one launch in k of each stage
+// whether this launch of stage st is timed (counts the stage's launches)
+bool prof_take(int st);
 void prof_reset();
 void prof_begin(int st, hipStream_t s);
 void prof_end(int st, hipStream_t s);
@@ -27,7 +30,7 @@ struct ProfScope {
   int st;
   hipStream_t s;
   bool on;
-  ProfScope(int st_, hipStream_t s_) : st(st_), s(s_), on(prof_stage_on(st_)) {
+  ProfScope(int st_, hipStream_t s_) : st(st_), s(s_), on(prof_stage_on(st_) && prof_take(st_)) {
     if (on) prof_begin(st, s);
   }
   ~ProfScope() {

@@ -410,6 +410,10 @@ int fresco_prof_enable(int on);
 /* Time only the stages whose bit (1 << stage) is set (0 = off): fewer events
  * on the stream when one kernel's duration is wanted. */
 int fresco_prof_enable_stages(unsigned mask);
+/* Time one launch of every `period` of each enabled stage (1 = every launch,
+ * the default): the stream carries fewer events when a kernel's average
+ * duration is sampled during a throughput run. */
+int fresco_prof_sample_period(int period);
 int fresco_prof_reset(void);
 int fresco_prof_get(int stage, const char** name, double* total_ms, int64_t* count);
 
