@@ -350,6 +350,15 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad, d
   });
 }
 
+int fresco_pot_energy_gradient_stream(fresco_pot* pot, const double* x, double* grad, double* energy,
+                                      void* stream) {
+  return guarded([&]() {
+    auto* p = checked(pot);
+    if (!x || !grad || !energy) throw fresco_hip::InvalidArgument("null pointer");
+    *energy = p->energy_gradient_stream(x, grad, static_cast<hipStream_t>(stream));
+  });
+}
+
 int fresco_pot_energy_keep(fresco_pot* pot, const double* x, double* energy, uint64_t* token, void* stream) {
   return guarded([&]() {
     auto* p = checked(pot);
@@ -440,6 +449,10 @@ int fresco_prof_reset(void) {
 int fresco_prof_get(int stage, const char** name, double* total_ms, int64_t* count) {
   return guarded([&]() {
     if (stage < 0 || stage >= fresco_hip::ST_COUNT) throw fresco_hip::InvalidArgument("stage out of range");
+    // every recorded pair complete (evaluations may return before their
+    // streams drain)
+    FRESCO_HIP_CHECK(hipDeviceSynchronize());
+    fresco_hip::prof_collect();
     long long c = 0;
     fresco_hip::prof_get(stage, total_ms, &c);
     *count = c;

@@ -245,6 +245,14 @@ class NUwU final : public Potential {
     return host_.wait(s);
   }
 
+  // the energy reaches the host before the interpolation (and, in the fused
+  // band form, most of the inverse transform) has run
+  double energy_gradient_stream(const double* x, double* grad, hipStream_t s) override {
+    host_.arm();
+    enqueue_eg(x, grad, e_dev_.get(), host_.dev(), s);
+    return host_.wait_value(s);
+  }
+
   bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
     enqueue_eg(x, grad, e_dev, nullptr, s);
     return true;

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -22,6 +23,12 @@ class Potential {
   virtual double energy(const double* x, hipStream_t s) = 0;
   // base_potential.hpp:38-43; grad (device, ndof) is overwritten
   virtual double energy_gradient(const double* x, double* grad, hipStream_t s) = 0;
+  // energy_gradient that returns once the energy has reached the host; the
+  // rest of the gradient pass may still be running on s (stream-ordered
+  // callers).  Potentials without that form synchronise fully.
+  virtual double energy_gradient_stream(const double* x, double* grad, hipStream_t s) {
+    return energy_gradient(x, grad, s);
+  }
   // energy_gradient without the wait: enqueues the evaluation on s and
   // leaves the energy in *e_dev (device).  Returns false (enqueuing nothing)
   // when the potential has no such path.  What it enqueues is kernels only
@@ -131,8 +138,31 @@ class HostScalar {
     if (prof_enabled()) prof_collect();
     return *p_;
   }
+  // before enqueuing a kernel that writes it: a marker the kernel's value
+  // replaces (a NaN payload no arithmetic produces)
+  void arm() { *reinterpret_cast<volatile uint64_t*>(p_) = kMarker; }
+  // the value, as soon as the kernel (enqueued on s after arm()) has written
+  // it: the host polls the pinned word instead of waiting for the whole
+  // stream; if the stream drains without the value (a failed launch), the
+  // stream is synchronised, which reports the error.  Stage-profiler events
+  // completed so far are collected (fresco_prof_get collects the rest).
+  double wait_value(hipStream_t s) {
+    const volatile uint64_t* w = reinterpret_cast<const volatile uint64_t*>(p_);
+    for (unsigned it = 1;; ++it) {
+      if (*w != kMarker) break;
+      if ((it & 255u) == 0u && hipStreamQuery(s) != hipErrorNotReady) {
+        FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    if (prof_enabled()) prof_collect();
+    return *p_;
+  }
+
 
  private:
+  static constexpr uint64_t kMarker = 0x7ff4f3e5c0de0000ull;
   double* p_ = nullptr;
   double* d_ = nullptr;
 };

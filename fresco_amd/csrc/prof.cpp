@@ -91,7 +91,14 @@ void prof_end(int st, hipStream_t s) {
 
 void prof_collect() {
   std::lock_guard<std::mutex> l(g_mu);
+  // pairs whose end event has not completed stay pending (their events are
+  // not reused)
+  std::vector<Pending> keep;
   for (auto& p : g_pending) {
+    if (hipEventQuery(p.b) != hipSuccess) {
+      keep.push_back(p);
+      continue;
+    }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
       g_ms[p.st] += ms;
@@ -100,7 +107,7 @@ void prof_collect() {
     g_pool.push_back(p.a);
     g_pool.push_back(p.b);
   }
-  g_pending.clear();
+  g_pending.swap(keep);
 }
 
 void prof_get(int st, double* total_ms, long long* count) {

@@ -80,8 +80,10 @@ class BasePotential:
         self._last_x = xt
         g = torch.empty_like(xt)
         e = ctypes.c_double()
+        # stream-ordered: the gradient is completed by work on the current
+        # stream, which the tensor's uses (and .cpu()) follow
         _lib.check(
-            lib.fresco_pot_energy_gradient(
+            lib.fresco_pot_energy_gradient_stream(
                 self._h, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(g.data_ptr()), ctypes.byref(e),
                 stream_handle(),
             )
@@ -98,13 +100,16 @@ class BasePotential:
         return self.get_energy_gradient(x)
 
     def energy_gradient_device(self, x, grad, stream=None):
-        """Zero-copy call: x, grad are contiguous float64 device tensors."""
+        """Zero-copy call: x, grad are contiguous float64 device tensors.  The
+        energy is returned as soon as it is known; grad is complete for work
+        ordered after the call on the stream used (the current stream, or
+        `stream`), as for any asynchronous torch operation."""
         lib = _lib.load()
         self._last_x = x
         e = ctypes.c_double()
         s = stream_handle() if stream is None else ctypes.c_void_p(stream.cuda_stream)
         _lib.check(
-            lib.fresco_pot_energy_gradient(
+            lib.fresco_pot_energy_gradient_stream(
                 self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(grad.data_ptr()), ctypes.byref(e), s
             )
         )

@@ -202,6 +202,19 @@ int fresco_pot_energy_gradient(fresco_pot* pot, const double* x, double* grad,
                                double* energy, void* stream);
 
 /*
+ * fresco_pot_energy_gradient for stream-ordered callers: returns once the
+ * energy is on the host; the rest of the gradient pass may still be running
+ * on `stream`, so grad is read after work ordered on that stream (torch
+ * tensors on the caller's current stream; the next evaluation's kernels
+ * queue behind it instead of waiting for the host).  Potentials without
+ * that form synchronise as fresco_pot_energy_gradient.  No reference
+ * counterpart (the reference's get_energy_gradient is host-synchronous,
+ * base_potential.hpp:38-43).
+ */
+int fresco_pot_energy_gradient_stream(fresco_pot* pot, const double* x, double* grad,
+                                      double* energy, void* stream);
+
+/*
  * The energy of fresco_pot_energy_gradient at x (device, float64), keeping the
  * state its gradient needs (for NUwU: the sorted points and the pullback
  * spectrum) so that fresco_pot_gradient_kept can finish the pass later: the
