@@ -118,7 +118,9 @@ __global__ void k_pattern_sums(const double* __restrict__ partial, const int* __
   }
 }
 
-__global__ void k_total(const double* __restrict__ epat, int npat, double* __restrict__ total) {
+// total_host (device-visible pinned host memory, or null) receives it too
+__global__ void k_total(const double* __restrict__ epat, int npat, double* __restrict__ total,
+                        double* __restrict__ total_host) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     double s = 0.0, c = 0.0;
     for (int p = 0; p < npat; ++p) {
@@ -128,6 +130,10 @@ __global__ void k_total(const double* __restrict__ epat, int npat, double* __res
       s = t;
     }
     total[0] = s;
+    if (total_host) {
+      total_host[0] = s;
+      __threadfence_system();
+    }
   }
 }
 
@@ -331,6 +337,14 @@ class BatchNUwU final : public Potential {
     return host_.fetch(e_.get(), s);
   }
 
+  // the energy reaches the host before the inverse transforms and the
+  // interpolation have run
+  double energy_gradient_stream(const double* x, double* grad, hipStream_t s) override {
+    host_.arm();
+    enqueue_eg(x, grad, e_.get(), s, host_.dev());
+    return host_.wait_value(s);
+  }
+
   bool energy_gradient_async(const double* x, double* grad, double* e_dev, hipStream_t s) override {
     enqueue_eg(x, grad, e_dev, s);
     return true;
@@ -353,7 +367,7 @@ class BatchNUwU final : public Potential {
   }
 
  private:
-  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s) {
+  void enqueue_eg(const double* x, double* grad, double* e_out, hipStream_t s, double* e_host = nullptr) {
     forward(x, map_g_, s);
     {
       ProfScope p(ST_ZERO, s);
@@ -364,7 +378,7 @@ class BatchNUwU final : public Potential {
       hipLaunchKernelGGL((k_batch_loss<T, 1>), dim3(nblk_), dim3(BT), 0, s, S_.get(), pd_, X_.get(),
                          partial_.get());
       FRESCO_HIP_CHECK(hipGetLastError());
-      finish_energy(s, e_out);
+      finish_energy(s, e_out, e_host);
     }
     {
       ProfScope p(ST_FFT_INV, s);
@@ -386,10 +400,10 @@ class BatchNUwU final : public Potential {
     r2c_.exec(grid_.get(), S_.get(), s);
   }
 
-  void finish_energy(hipStream_t s, double* e_out) {
+  void finish_energy(hipStream_t s, double* e_out, double* e_host = nullptr) {
     hipLaunchKernelGGL(k_pattern_sums, dim3((P_ + 127) / 128), dim3(128), 0, s, partial_.get(), pblk0_.get(), P_,
                        epat_.get(), e_out);
-    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, epat_.get(), P_, e_out);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, epat_.get(), P_, e_out, e_host);
     FRESCO_HIP_CHECK(hipGetLastError());
   }
 
