@@ -152,6 +152,7 @@ class HostScalar {
       if (*w != kMarker) break;
       if ((it & 255u) == 0u && hipStreamQuery(s) != hipErrorNotReady) {
         FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+        if (*w == kMarker) throw RuntimeFailure("the evaluation finished without delivering its energy");
         break;
       }
     }

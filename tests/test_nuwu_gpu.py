@@ -105,6 +105,36 @@ def test_deterministic(NUwU):
         assert np.array_equal(g1, g2)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_stream_ordered_evaluation_matches_synchronous(NUwU, prec):
+    """energy_gradient_device returns once the energy is on the host (the
+    gradient completes on the stream, read after it): back-to-back
+    evaluations queue behind each other, and their energies and gradients
+    equal the host-synchronous C-ABI call's bitwise (3D, band form for FP32)."""
+    import ctypes
+
+    import torch
+
+    from fresco_amd import _lib
+    from fresco_amd._device import stream_handle
+
+    radii, Sk, V, p = make_case(3, 21, 6000, seed=17)
+    pot = NUwU(radii, 0.0, Sk, V, np.ones(3), EPS[prec], precision=prec)
+    x = torch.tensor(p, device="cuda")
+    gs = [torch.empty_like(x) for _ in range(6)]
+    es = [pot.energy_gradient_device(x, g) for g in gs]
+    g0 = torch.empty_like(x)
+    e0 = ctypes.c_double()
+    _lib.check(_lib.load().fresco_pot_energy_gradient(
+        pot._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(g0.data_ptr()), ctypes.byref(e0), stream_handle()))
+    torch.cuda.synchronize()
+    assert all(e == e0.value for e in es)
+    assert all(torch.equal(g, g0) for g in gs)
+    E, g = O.NUwU(radii, 0.0, Sk, V, np.ones(3)).get_energy_gradient(p)
+    assert abs(e0.value - E) / abs(E) < TOL[prec]
+    assert rel(g0.cpu().numpy(), g) < TOL[prec]
+
+
 def test_torch_tensor_in_device_tensor_out(NUwU):
     import torch
 
