@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -149,11 +150,16 @@ class HostScalar {
   double wait_value(hipStream_t s) {
     const volatile uint64_t* w = reinterpret_cast<const volatile uint64_t*>(p_);
     for (unsigned it = 1;; ++it) {
-      if (*w != kMarker) break;
-      if ((it & 255u) == 0u && hipStreamQuery(s) != hipErrorNotReady) {
-        FRESCO_HIP_CHECK(hipStreamSynchronize(s));
-        if (*w == kMarker) throw RuntimeFailure("the evaluation finished without delivering its energy");
-        break;
+      const uint64_t a = *w;
+      // two equal reads: the value is whole (not read while being written)
+      if (a != kMarker && a == *w) break;
+      if ((it & 255u) == 0u) {
+        if (hipStreamQuery(s) != hipErrorNotReady) {
+          FRESCO_HIP_CHECK(hipStreamSynchronize(s));
+          if (*w == kMarker) throw RuntimeFailure("the evaluation finished without delivering its energy");
+          break;
+        }
+        if (it >= 4096u) std::this_thread::yield();
       }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
