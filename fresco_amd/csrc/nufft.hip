@@ -228,10 +228,16 @@ struct MapDev {
   int d, wrap, w;
   double L[3], shift[3], scale[3];
   int nf[3], tile[3], ntile[3];
+  int tshift[3];     // log2 tile[a] for power-of-two tiles, else -1
+  double itile[3];   // 1 / tile[a]
   int cell_bits;
 };
 
 __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, int& cell, double& off) {
+  // no fused multiply-adds: every kernel that maps a point gets the same
+  // cell and offset bit for bit (the direct binning counts in one kernel and
+  // places in another, both mapping the point)
+#pragma clang fp contract(off)
   double xs = m.wrap ? (v - round(v / m.L[a]) + m.shift[a]) * m.scale[a] : v * m.scale[a];
   double t = xs * (1.0 / (2.0 * M_PI));
   t -= floor(t);
@@ -255,19 +261,36 @@ __device__ __forceinline__ void map_component(double v, const MapDev& m, int a, 
 }
 
 // bin key of point j (b0 + tile of its first support cell), its record
-// (fine-grid offsets and weight) and its packed first cell
-template <typename T>
-__device__ __forceinline__ uint32_t point_record(const double* __restrict__ x, int64_t j, const MapDev& m,
-                                                 uint32_t b0, const T* __restrict__ c, uint32_t& packed,
-                                                 typename Vec4<T>::type& r) {
+// (fine-grid offsets and weight) and its packed first cell; the D
+// coordinates are loaded together, the map is unrolled over them
+template <typename T, int D>
+__device__ __forceinline__ uint32_t point_record_d(const double* __restrict__ x, int64_t j, const MapDev& m,
+                                                   uint32_t b0, const T* __restrict__ c, uint32_t& packed,
+                                                   typename Vec4<T>::type& r) {
+  double v[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) v[a] = x[j * D + a];
+  const T wt = c ? c[j] : T(1);
   uint32_t b = b0, mul = 1;
   double offs[3] = {0, 0, 0};
   packed = 0;
-  for (int a = 0; a < m.d; ++a) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
     int cell;
-    map_component(x[j * m.d + a], m, a, cell, offs[a]);
+    map_component(v[a], m, a, cell, offs[a]);
     packed |= (uint32_t)cell << (a * m.cell_bits);
-    int t = cell / m.tile[a];
+    // cell / tile[a] without an integer division: a shift, else the
+    // float64 quotient corrected by one step (cell < 2^31: off by one at most)
+    int t;
+    if (m.tshift[a] >= 0) {
+      t = cell >> m.tshift[a];
+    } else {
+      t = (int)((double)cell * m.itile[a]);
+      if (t * m.tile[a] > cell)
+        --t;
+      else if ((t + 1) * m.tile[a] <= cell)
+        ++t;
+    }
     t = t < m.ntile[a] ? t : m.ntile[a] - 1;
     b += mul * (uint32_t)t;
     mul *= (uint32_t)m.ntile[a];
@@ -275,8 +298,17 @@ __device__ __forceinline__ uint32_t point_record(const double* __restrict__ x, i
   r.x = (T)offs[0];
   r.y = (T)offs[1];
   r.z = (T)offs[2];
-  r.w = c ? c[j] : T(1);
+  r.w = wt;
   return b;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t point_record(const double* __restrict__ x, int64_t j, const MapDev& m,
+                                                 uint32_t b0, const T* __restrict__ c, uint32_t& packed,
+                                                 typename Vec4<T>::type& r) {
+  if (m.d == 3) return point_record_d<T, 3>(x, j, m, b0, c, packed, r);
+  if (m.d == 2) return point_record_d<T, 2>(x, j, m, b0, c, packed, r);
+  return point_record_d<T, 1>(x, j, m, b0, c, packed, r);
 }
 
 // bin key of every point (pattern * nbins + tile of its first support cell)
@@ -300,9 +332,9 @@ __global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m,
 // ---------------------------------------------------------------------------
 // Direct binning: one counting pass when all bins of the plan (batch * nbins)
 // fit PB_MAXB and the plan is not segmented.
-//   k_bin_tiles   per tile of PB_TILE points: key, record and packed cell in
-//                 the original order, and the tile's bin counts (tile-major,
-//                 cnt[tile * nb + bin], so every later access is coalesced)
+//   k_bin_tiles   per tile of PB_TILE points: the tile's bin counts
+//                 (tile-major, cnt[tile * nb + bin], so every later access is
+//                 coalesced); nothing per point is stored
 //   k_bin_groups  per group of gsz tiles and bin: exclusive prefix of the
 //                 counts over the group's tiles, in place; the group total
 //                 to gsum[group * nb + bin]
@@ -310,9 +342,14 @@ __global__ void k_bin_records(const double* __restrict__ x, int64_t n, MapDev m,
 //                 the groups (in place) and of the bin totals over the bins
 //                 (the bin starts)
 //   k_bin_place   per tile: running offsets start + group prefix + tile
-//                 prefix, then rounds of 256 points in index order, each
-//                 wavefront ranking its lanes among equal bins with one
-//                 ballot per key bit; records move to their sorted places
+//                 prefix; each wavefront ranks its quarter of the tile
+//                 (rounds of 64 points in index order, one ballot per key
+//                 bit, a running count per bin in LDS), then every point
+//                 goes to offset + earlier wavefronts' counts + rank (two
+//                 barriers per tile); each point is mapped again (reading
+//                 the coordinates twice costs less than writing and reading
+//                 back records in the original order) and its record, cell
+//                 and index go to their sorted places
 // The result is the stable sort by key (index order inside every bin) that
 // the radix sort gives, without its extra key and index arrays.
 // ---------------------------------------------------------------------------
@@ -322,36 +359,30 @@ constexpr int PB_TILE = PB_T * PB_ROUNDS;
 constexpr int PB_MAXB = 2048;
 constexpr int PB_GMAX = 32;  // at most this many tile groups
 
-constexpr int PB_CT = 1024;  // threads of k_bin_tiles: PB_TILE / PB_CT points each
+constexpr int PB_CT = 512;  // threads of k_bin_tiles: PB_TILE / PB_CT points each (loads in flight)
 
 template <typename T>
 __global__ void __launch_bounds__(PB_CT)
     k_bin_tiles(const double* __restrict__ x, int64_t n, MapDev m, const uint32_t* __restrict__ pat,
-                uint32_t nbins, const T* __restrict__ c, int nb, uint32_t* __restrict__ key,
-                uint32_t* __restrict__ cell_out, typename Vec4<T>::type* __restrict__ rec_out,
-                uint32_t* __restrict__ cnt) {
+                uint32_t nbins, int nb, uint32_t* __restrict__ cnt) {
   constexpr int PER = PB_TILE / PB_CT;
   __shared__ uint32_t h[PB_MAXB];
   const int t = threadIdx.x;
   for (int b = t; b < nb; b += PB_CT) h[b] = 0;
   const int64_t base = (int64_t)blockIdx.x * PB_TILE;
-  uint32_t k[PER], packed[PER];
-  typename Vec4<T>::type rc[PER];
+  uint32_t k[PER];
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int64_t j = base + r * PB_CT + t;
-    if (j < n) k[r] = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, packed[r], rc[r]);
+    uint32_t packed;
+    typename Vec4<T>::type rc;
+    if (j < n) k[r] = point_record<T>(x, j, m, pat ? pat[j] * nbins : 0u, nullptr, packed, rc);
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int64_t j = base + r * PB_CT + t;
-    if (j < n) {
-      key[j] = k[r];
-      cell_out[j] = packed[r];
-      rec_out[j] = rc[r];
-      atomicAdd(&h[k[r]], 1u);
-    }
+    if (j < n) atomicAdd(&h[k[r]], 1u);
   }
   __syncthreads();
   for (int b = t; b < nb; b += PB_CT) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
@@ -423,8 +454,8 @@ __global__ void __launch_bounds__(PB_BT)
 
 template <typename T>
 __global__ void __launch_bounds__(PB_T)
-    k_bin_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ cell_in,
-                const typename Vec4<T>::type* __restrict__ rec_in, int64_t n, int nb, int dbits,
+    k_bin_place(const double* __restrict__ x, MapDev m, const uint32_t* __restrict__ pat, uint32_t nbins,
+                const T* __restrict__ c, int64_t n, int nb, int dbits,
                 const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ gsum, int gsz,
                 const int* __restrict__ start, uint32_t* __restrict__ idx, uint32_t* __restrict__ cell_out,
                 typename Vec4<T>::type* __restrict__ rec_out) {
@@ -446,50 +477,55 @@ __global__ void __launch_bounds__(PB_T)
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
-  const int64_t base = (int64_t)tile * PB_TILE;
-  // every round's inputs are loaded up front: the rounds then wait on LDS
-  // and barriers only
-  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS];
+  // wavefront wv owns points [base, base + 64 PB_ROUNDS) of the tile, in
+  // rounds of 64; all of them are mapped up front (the same key, cell and
+  // record k_bin_tiles counted: the map has no contractions)
+  const int64_t base = (int64_t)tile * PB_TILE + (int64_t)wv * (64 * PB_ROUNDS);
+  uint32_t ks[PB_ROUNDS], cls[PB_ROUNDS], rk[PB_ROUNDS];
   typename Vec4<T>::type rcs[PB_ROUNDS];
 #pragma unroll
   for (int r = 0; r < PB_ROUNDS; ++r) {
-    const int64_t j = base + r * PB_T + t;
+    const int64_t j = base + r * 64 + lane;
     ks[r] = 0;
     cls[r] = 0;
-    if (j < n) {
-      ks[r] = key[j];
-      cls[r] = cell_in[j];
-      rcs[r] = rec_in[j];
-    }
+    if (j < n) ks[r] = point_record(x, j, m, pat ? pat[j] * nbins : 0u, c, cls[r], rcs[r]);
   }
+  __syncthreads();
+  // rank of every point among the wavefront's earlier points of its bin:
+  // equal-key lanes by one ballot per key bit, earlier rounds by the
+  // wavefront's running count wc[wv][bin] (LDS operations of one wavefront
+  // complete in order, so no barrier is needed)
 #pragma unroll
   for (int r = 0; r < PB_ROUNDS; ++r) {
-    const int64_t j = base + r * PB_T + t;
-    const bool valid = j < n;
-    const uint32_t k = ks[r], cl = cls[r];
-    const typename Vec4<T>::type rc = rcs[r];
+    const bool valid = base + r * 64 + lane < n;
+    const uint32_t k = ks[r];
     uint64_t same = __ballot(valid);
     for (int b = 0; b < dbits; ++b) {
       const bool bit = (k >> b) & 1u;
       const uint64_t mk = __ballot(bit);
       same &= bit ? mk : ~mk;
     }
-    const int rank = __popcll(same & lt);
-    if (valid && rank == 0) wc[wv][k] = (uint32_t)__popcll(same);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[k] + (uint32_t)rank;
+    const uint32_t rank = (uint32_t)__popcll(same & lt);
+    const uint32_t before = valid ? wc[wv][k] : 0u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (valid && rank == 0) wc[wv][k] = before + (uint32_t)__popcll(same);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    rk[r] = before + rank;
+  }
+  __syncthreads();
+  // place: the tile's running offset, the earlier wavefronts' counts, the rank
+#pragma unroll
+  for (int r = 0; r < PB_ROUNDS; ++r) {
+    const int64_t j = base + r * 64 + lane;
+    if (j < n) {
+      const uint32_t k = ks[r];
+      uint32_t pos = run[k] + rk[r];
       for (int w = 0; w < wv; ++w) pos += wc[w][k];
+      if (pos >= (uint32_t)n) pos = (uint32_t)n - 1u;  // unreachable when the counts hold
       idx[pos] = (uint32_t)j;
-      cell_out[pos] = cl;
-      rec_out[pos] = rc;
+      cell_out[pos] = cls[r];
+      rec_out[pos] = rcs[r];
     }
-    __syncthreads();
-    if (valid && rank == 0) {
-      atomicAdd(&run[k], wc[wv][k]);
-      wc[wv][k] = 0;
-    }
-    __syncthreads();
   }
 }
 
@@ -752,6 +788,10 @@ static MapDev make_map_dev(const GridGeom& g, const EsParams& es, const CoordMap
     m.nf[a] = g.nf[a];
     m.tile[a] = g.tile[a];
     m.ntile[a] = g.ntile[a];
+    m.tshift[a] = -1;
+    m.itile[a] = 1.0 / g.tile[a];
+    for (int k = 0; k < 30; ++k)
+      if (g.tile[a] == (1 << k)) m.tshift[a] = k;
   }
   m.cell_bits = g.cell_bits;
   return m;
@@ -854,13 +894,13 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
   if (direct) {
     int dbits = 1;
     while ((1 << dbits) < tb) ++dbits;
-    hipLaunchKernelGGL(k_bin_tiles<T>, dim3(ntiles_), dim3(PB_CT), 0, s, x, n, m, pat, (uint32_t)geom.nbins, c, tb,
-                       key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
+    hipLaunchKernelGGL(k_bin_tiles<T>, dim3(ntiles_), dim3(PB_CT), 0, s, x, n, m, pat, (uint32_t)geom.nbins, tb,
+                       tcnt_.get());
     hipLaunchKernelGGL(k_bin_groups, dim3((tb + PB_T - 1) / PB_T, ngroups_), dim3(PB_T), 0, s, tcnt_.get(), ntiles_,
                        tb, gsz_, gsum_.get());
     hipLaunchKernelGGL(k_bin_bases, dim3(1), dim3(PB_BT), 0, s, gsum_.get(), ngroups_, tb, n, bin_start.get());
-    hipLaunchKernelGGL(k_bin_place<T>, dim3(ntiles_), dim3(PB_T), 0, s, key.get(), scell_in.get(), rec_in.get(), n,
-                       tb, dbits, tcnt_.get(), gsum_.get(), gsz_, bin_start.get(), idx.get(), scell.get(), rec.get());
+    hipLaunchKernelGGL(k_bin_place<T>, dim3(ntiles_), dim3(PB_T), 0, s, x, m, pat, (uint32_t)geom.nbins, c, n, tb,
+                       dbits, tcnt_.get(), gsum_.get(), gsz_, bin_start.get(), idx.get(), scell.get(), rec.get());
     FRESCO_HIP_CHECK(hipGetLastError());
     return;
   }

@@ -428,12 +428,15 @@ def test_matrix_core_spreader_2d(NUwU, case, monkeypatch):
     assert rel(res["mfma"][1], res["halo"][1]) < 2e-6
 
 
-@pytest.mark.parametrize("d,n,N", [(3, 33, 5000), (3, 21, 2047), (2, 65, 4097), (1, 129, 3000)])
+@pytest.mark.parametrize(
+    "d,n,N", [(3, 33, 5000), (3, 21, 2047), (3, 41, 300007), (2, 65, 4097), (1, 129, 3000)]
+)
 def test_direct_binning_equals_radix_sort(NUwU, d, n, N, monkeypatch):
     """Plans whose bins fit one 11-bit digit are binned by one counting pass
     (tile counts, group prefixes, ballot-ranked placement) instead of the
     radix sort; both give index order inside every bin, so energy and
-    gradient are bitwise equal.  Sizes off the 2048-point tile."""
+    gradient are bitwise equal.  Sizes off the 1024-point tile and its
+    256-point wavefront quarters, up to 293 tiles in several groups."""
     radii, Sk, V, p = make_case(d, n, N, seed=60 + d)
     out = {}
     for mode in ("1", "0"):
