@@ -535,43 +535,38 @@ __global__ void __launch_bounds__(PB_T)
 // straddle a pattern (tile t: points [tbeg[t], tbeg[t + 1]) of pattern
 // tpat[t]), so counts, prefixes and places are per pattern over its local
 // bins; the result is the same stable order as the segmented radix sort.
-//   k_bin_tiles_seg  per tile: records in the original order and the
-//                    tile's local bin counts (cnt[tile * nbins + bin])
+//   k_bin_tiles_seg  per tile: the tile's local bin counts
+//                    (cnt[tile * nbins + bin]) from the mapped coordinates
 //   k_bin_scan_seg   per pattern: exclusive prefix of each local bin's
 //                    counts over the pattern's tiles (in place), then of the
 //                    bin totals over the bins: bin_start
-//   k_bin_place_seg  per tile: k_bin_place over local bins
+//   k_bin_place_seg  per tile: maps the points again, ranks them over
+//                    local bins and writes each bin's run from LDS
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(PB_CT)
-    k_bin_tiles_seg(const double* __restrict__ x, MapDev m, const uint32_t* __restrict__ pat, uint32_t nbins,
-                    const T* __restrict__ c, const int* __restrict__ tbeg, const int* __restrict__ tpat,
-                    uint32_t* __restrict__ key, uint32_t* __restrict__ cell_out,
-                    typename Vec4<T>::type* __restrict__ rec_out, uint32_t* __restrict__ cnt) {
+    k_bin_tiles_seg(const double* __restrict__ x, MapDev m, uint32_t nbins, const int* __restrict__ tbeg,
+                    uint32_t* __restrict__ cnt) {
   constexpr int PER = PB_TILE / PB_CT;
   __shared__ uint32_t h[PB_MAXB];
   const int t = threadIdx.x;
   const int nb = (int)nbins;
   for (int b = t; b < nb; b += PB_CT) h[b] = 0;
   const int j0 = tbeg[blockIdx.x], j1 = tbeg[blockIdx.x + 1];
-  const uint32_t base = (uint32_t)tpat[blockIdx.x] * nbins;
-  uint32_t k[PER], packed[PER];
-  typename Vec4<T>::type rc[PER];
+  // local bins (the tile lies inside one pattern): key offset 0
+  uint32_t k[PER];
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int j = j0 + r * PB_CT + t;
-    if (j < j1) k[r] = point_record(x, j, m, pat[j] * nbins, c, packed[r], rc[r]);
+    uint32_t packed;
+    typename Vec4<T>::type rc;
+    if (j < j1) k[r] = point_record<T>(x, j, m, 0u, nullptr, packed, rc);
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int j = j0 + r * PB_CT + t;
-    if (j < j1) {
-      key[j] = k[r];
-      cell_out[j] = packed[r];
-      rec_out[j] = rc[r];
-      atomicAdd(&h[k[r] - base], 1u);
-    }
+    if (j < j1) atomicAdd(&h[k[r]], 1u);
   }
   __syncthreads();
   for (int b = t; b < nb; b += PB_CT) cnt[(int64_t)blockIdx.x * nb + b] = h[b];
@@ -641,8 +636,7 @@ __host__ __device__ constexpr size_t place_seg_lds_bytes(int nbins) {
 
 template <typename T>
 __global__ void __launch_bounds__(PB_T)
-    k_bin_place_seg(const uint32_t* __restrict__ key, const uint32_t* __restrict__ cell_in,
-                    const typename Vec4<T>::type* __restrict__ rec_in, const int* __restrict__ tbeg,
+    k_bin_place_seg(const double* __restrict__ x, MapDev m, const T* __restrict__ c, const int* __restrict__ tbeg,
                     const int* __restrict__ tpat, int nbins, int dbits, const uint32_t* __restrict__ cnt,
                     const int* __restrict__ start, uint32_t* __restrict__ idx, uint32_t* __restrict__ cell_out,
                     typename Vec4<T>::type* __restrict__ rec_out) {
@@ -664,7 +658,6 @@ __global__ void __launch_bounds__(PB_T)
   const int nt = gridDim.x, xq = nt >> 3, xr = nt & 7, xc = blockIdx.x & 7;
   const int tile = xc * xq + (xc < xr ? xc : xr) + (blockIdx.x >> 3);
   const int64_t trow = (int64_t)tile * nbins;
-  const uint32_t kbase = (uint32_t)tpat[tile] * (uint32_t)nbins;
   const int64_t sbase = (int64_t)tpat[tile] * nbins;
   for (int b = t; b < nbins; b += PB_T) {
     const uint32_t r0 = (uint32_t)start[sbase + b] + cnt[trow + b];
@@ -683,11 +676,9 @@ __global__ void __launch_bounds__(PB_T)
     const int j = j0 + r * PB_T + t;
     ks[r] = 0;
     cls[r] = 0;
-    if (j < j1) {
-      ks[r] = key[j] - kbase;
-      cls[r] = cell_in[j];
-      rcs[r] = rec_in[j];
-    }
+    // mapped again: the local bin k_bin_tiles_seg counted (the map has no
+    // contractions), the record and the packed cell
+    if (j < j1) ks[r] = point_record(x, j, m, 0u, c, cls[r], rcs[r]);
   }
   // each point's place: rounds of 256 points in index order, ranks among
   // equal bins by ballots
@@ -907,13 +898,12 @@ void SortedPoints<T>::set(const double* x, const CoordMap& cm, const T* c, hipSt
   if (seg_direct_) {
     int dbits = 1;
     while ((1 << dbits) < geom.nbins) ++dbits;
-    hipLaunchKernelGGL(k_bin_tiles_seg<T>, dim3(seg_ntiles_), dim3(PB_CT), 0, s, x, m, pat, (uint32_t)geom.nbins, c,
-                       seg_tbeg_.get(), seg_tpat_.get(), key.get(), scell_in.get(), rec_in.get(), tcnt_.get());
+    hipLaunchKernelGGL(k_bin_tiles_seg<T>, dim3(seg_ntiles_), dim3(PB_CT), 0, s, x, m, (uint32_t)geom.nbins,
+                       seg_tbeg_.get(), tcnt_.get());
     hipLaunchKernelGGL(k_bin_scan_seg, dim3(geom.batch), dim3(PB_SGT), 0, s, tcnt_.get(), seg_ptile_.get(),
                        seg_pstart_.get(), geom.nbins, geom.batch, n, bin_start.get());
     const size_t lds = place_seg_lds_bytes<typename Vec4<T>::type>(geom.nbins);
-    hipLaunchKernelGGL(k_bin_place_seg<T>, dim3(seg_ntiles_), dim3(PB_T), lds, s, key.get(), scell_in.get(),
-                       rec_in.get(), seg_tbeg_.get(), seg_tpat_.get(), geom.nbins, dbits, tcnt_.get(), bin_start.get(),
+    hipLaunchKernelGGL(k_bin_place_seg<T>, dim3(seg_ntiles_), dim3(PB_T), lds, s, x, m, c, seg_tbeg_.get(), seg_tpat_.get(), geom.nbins, dbits, tcnt_.get(), bin_start.get(),
                        idx.get(), scell.get(), rec.get());
     FRESCO_HIP_CHECK(hipGetLastError());
     return;
