@@ -800,9 +800,6 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
   bin_start.allocate(tb + 1);
   scell.allocate(n);
   rec.allocate(n);
-  scell_in.allocate(n);
-  rec_in.allocate(n);
-  key.allocate(n);
   // FRESCO_DIRECT_BINNING=0: always the radix sort (the same order)
   // interpolation layout: FRESCO_INTERP_LPP=8 (eight lanes per point instead
   // of four), FRESCO_INTERP_WGS (target workgroup count, default 2048),
@@ -831,6 +828,11 @@ void SortedPoints<T>::init(const GridGeom& g, const EsParams& e, int64_t npts) {
 template <typename T>
 void SortedPoints<T>::init_sorter() {
   direct = false;
+  // keys and the records in the original order: the radix sort's input
+  // (the direct passes map the points twice instead)
+  scell_in.allocate(n);
+  rec_in.allocate(n);
+  key.allocate(n);
   idx_in.allocate(n);
   key_sorted.allocate(n);
   sorter.init(n, geom.key_bits);

@@ -99,8 +99,8 @@ class SortedPoints {
   DeviceBuffer<int> bin_start;                          // [batch * nbins + 1]
   DeviceBuffer<uint32_t> scell;                         // packed first cell per sorted point
   DeviceBuffer<R4> rec;                                 // (off_x, off_y, off_z, c) per sorted point
-  DeviceBuffer<uint32_t> scell_in;                      // the same in the original order
-  DeviceBuffer<R4> rec_in;
+  DeviceBuffer<uint32_t> scell_in;                      // the same in the original order (radix sort
+  DeviceBuffer<R4> rec_in;                              // input; key too: allocated for that path only)
 
  private:
   void init_sorter();
